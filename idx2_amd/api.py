@@ -1,0 +1,231 @@
+"""Python binding of libidx2g (include/idx2g.h) over ctypes.
+
+Mirrors the reference's library surface (Source/idx2.h): encode a grid,
+open a dataset, ask for the output grid of a query, decode into a buffer.
+Device buffers are plain integer addresses (e.g. torch.Tensor.data_ptr()).
+The library must have been built (idx2_amd/build.py); there is no fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libidx2g.so")
+
+ERRORS = {
+    0: "NoError", 1: "Unknown", 2: "SizeZero", 3: "SizeTooSmall", 4: "DimensionMismatched",
+    5: "SizeMismatched", 6: "AlignmentError", 7: "OutOfMemory", 8: "TypeNotSupported",
+    11: "FileReadFailed", 12: "FileWriteFailed", 13: "FileCreateFailed", 17: "ParseFailed",
+    21: "BrickSizeNotPowerOfTwo", 22: "BrickSizeTooBig", 23: "TooManyLevels",
+    27: "TooManyFilesPerDir", 28: "NotSupportedInVersion", 29: "CannotCreateDirectory",
+    30: "SyntaxError", 31: "TooManyBricksPerChunk", 32: "TooManyChunksPerFile",
+    33: "ChunksPerFileNotPowerOf2", 34: "BricksPerChunkNotPowerOf2", 35: "ChunkNotFound",
+    36: "BrickNotFound", 37: "FileNotFound", 100: "DeviceError",
+}
+
+
+class Idx2Error(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)} ({code}) {msg}".strip())
+
+
+class EncodeParams(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("field", ctypes.c_char_p), ("dims", ctypes.c_int32 * 3),
+                ("dtype", ctypes.c_int32), ("brick", ctypes.c_int32 * 3), ("nlevels", ctypes.c_int32),
+                ("tolerance", ctypes.c_double), ("bricks_per_chunk", ctypes.c_int32),
+                ("chunks_per_file", ctypes.c_int32), ("files_per_dir", ctypes.c_int32),
+                ("bit_planes_per_chunk", ctypes.c_int32)]
+
+
+class Query(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_int32 * 3), ("last", ctypes.c_int32 * 3),
+                ("downsampling", ctypes.c_int32 * 3), ("tolerance", ctypes.c_double)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("transform_ms", ctypes.c_double),
+                ("assemble_ms", ctypes.c_double), ("dominant_launches", ctypes.c_int64),
+                ("dominant_ms", ctypes.c_double), ("raw_bytes", ctypes.c_int64),
+                ("chunk_bytes", ctypes.c_int64), ("exp_bytes", ctypes.c_int64)]
+
+
+EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_encode",
+           "idx2g_dataset_write", "idx2g_open", "idx2g_dataset_free", "idx2g_dataset_info",
+           "idx2g_output_dims", "idx2g_decode", "idx2g_last_stats", "idx2g_dataset_chunk_bytes",
+           "idx2g_synth_field"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with idx2_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.idx2g_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int]
+        L.idx2g_destroy.argtypes = [vp]
+        L.idx2g_last_error_message.restype = ctypes.c_char_p
+        L.idx2g_encode.argtypes = [vp, ctypes.POINTER(EncodeParams), vp, ctypes.c_int, ctypes.POINTER(vp)]
+        L.idx2g_dataset_write.argtypes = [vp, ctypes.c_char_p]
+        L.idx2g_open.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.idx2g_dataset_free.argtypes = [vp]
+        L.idx2g_dataset_info.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32),
+                                         ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
+        L.idx2g_output_dims.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.idx2g_decode.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.c_int]
+        L.idx2g_last_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.idx2g_dataset_chunk_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(i64)]
+        L.idx2g_synth_field.argtypes = [vp, vp, ctypes.POINTER(i32), i32, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.c_uint64, i32, i32]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise Idx2Error(rc, lib().idx2g_last_error_message().decode(errors="replace"))
+
+
+def make_query(first=None, last=None, downsampling=(0, 0, 0), tolerance=0.0):
+    q = Query()
+    f = first if first is not None else (-1, -1, -1)
+    l_ = last if last is not None else (-1, -1, -1)
+    for d in range(3):
+        q.first[d] = int(f[d])
+        q.last[d] = int(l_[d])
+        q.downsampling[d] = int(downsampling[d])
+    q.tolerance = float(tolerance)
+    return q
+
+
+class Context:
+    """One GPU, one HIP stream (idx2g_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        _check(lib().idx2g_create(ctypes.byref(self._h), device))
+
+    def close(self):
+        if self._h:
+            lib().idx2g_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        _check(lib().idx2g_last_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def synth_field(self, dev_ptr, dims, dtype, seed, z_range=None):
+        from . import synth
+        prm = synth.field_params(seed)
+        arr = (ctypes.c_double * 192)(*prm.tolist())
+        d3 = (ctypes.c_int32 * 3)(*[int(x) for x in dims])
+        z0, z1 = (0, int(dims[2])) if z_range is None else z_range
+        _check(lib().idx2g_synth_field(self._h, ctypes.c_void_p(dev_ptr), d3, 1 if dtype == "float64" else 0, arr,
+                                       seed + 0x5EED, z0, z1))
+
+
+class Dataset:
+    """A dataset resident on the GPU (idx2g_dataset)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self._h = handle
+
+    @classmethod
+    def encode(cls, ctx, data=None, *, dev_ptr=None, dims=None, dtype=None, name="data", field="field",
+               brick=(32, 32, 32), nlevels=1, tolerance=1e-3, bricks_per_chunk=4096, chunks_per_file=64,
+               files_per_dir=64):
+        """Encode a numpy array of shape (nz, ny, nx), or a device buffer."""
+        p = EncodeParams()
+        p.name = name.encode()
+        p.field = field.encode()
+        if data is not None:
+            data = np.ascontiguousarray(data)
+            nz, ny, nx = data.shape
+            dims = (nx, ny, nz)
+            dtype = "float64" if data.dtype == np.float64 else "float32"
+            ptr, is_dev = data.ctypes.data_as(ctypes.c_void_p), 0
+        else:
+            ptr, is_dev = ctypes.c_void_p(dev_ptr), 1
+        for d in range(3):
+            p.dims[d] = int(dims[d])
+            p.brick[d] = int(brick[d])
+        p.dtype = 1 if dtype in ("float64", np.float64) else 0
+        p.nlevels = nlevels
+        p.tolerance = tolerance
+        p.bricks_per_chunk = bricks_per_chunk
+        p.chunks_per_file = chunks_per_file
+        p.files_per_dir = files_per_dir
+        p.bit_planes_per_chunk = 1
+        h = ctypes.c_void_p()
+        _check(lib().idx2g_encode(ctx._h, ctypes.byref(p), ptr, is_dev, ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def open(cls, ctx, idx2_path, in_dir=None):
+        h = ctypes.c_void_p()
+        if in_dir is None:
+            in_dir = os.path.dirname(os.path.dirname(os.path.abspath(idx2_path)))
+        _check(lib().idx2g_open(ctx._h, idx2_path.encode(), in_dir.encode(), ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def write(self, out_dir):
+        _check(lib().idx2g_dataset_write(self._h, out_dir.encode()))
+
+    def free(self):
+        if self._h:
+            lib().idx2g_dataset_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self):
+        d, b = (ctypes.c_int32 * 3)(), (ctypes.c_int32 * 3)()
+        dt, nl = ctypes.c_int32(), ctypes.c_int32()
+        tol, cb = ctypes.c_double(), ctypes.c_int64()
+        _check(lib().idx2g_dataset_info(self._h, d, ctypes.byref(dt), ctypes.byref(nl), b, ctypes.byref(tol),
+                                        ctypes.byref(cb)))
+        return dict(dims=tuple(d), dtype="float64" if dt.value else "float32", nlevels=nl.value, brick=tuple(b),
+                    tolerance=tol.value, chunk_bytes=cb.value)
+
+    def output_dims(self, **query):
+        q = make_query(**query)
+        dims = (ctypes.c_int64 * 3)()
+        nbytes = ctypes.c_int64()
+        _check(lib().idx2g_output_dims(self._h, ctypes.byref(q), dims, ctypes.byref(nbytes)))
+        return tuple(dims), nbytes.value
+
+    def decode(self, out_dev_ptr=None, **query):
+        """Decode a query; returns a numpy array (nz, ny, nx), or fills a device buffer."""
+        q = make_query(**query)
+        dims, nbytes = self.output_dims(**query)
+        if out_dev_ptr is not None:
+            _check(lib().idx2g_decode(self.ctx._h, self._h, ctypes.byref(q), ctypes.c_void_p(out_dev_ptr), 1))
+            return None
+        dt = np.float64 if self.info()["dtype"] == "float64" else np.float32
+        out = np.empty((dims[2], dims[1], dims[0]), dtype=dt)
+        _check(lib().idx2g_decode(self.ctx._h, self._h, ctypes.byref(q), out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out
+
+    def chunk_bytes(self):
+        n = ctypes.c_int64()
+        _check(lib().idx2g_dataset_chunk_bytes(self._h, None, ctypes.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint8)
+        _check(lib().idx2g_dataset_chunk_bytes(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                               ctypes.byref(n)))
+        return buf

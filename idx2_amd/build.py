@@ -1,0 +1,47 @@
+"""Build libidx2g.so (HIP, gfx950) in-tree: idx2_amd/_build/libidx2g.so."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+OUT_DIR = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT_DIR, "libidx2g.so")
+SRC = [os.path.join(HERE, "csrc", f) for f in ("idx2g.hip", "layout.cpp", "common.cpp")]
+DEPS = SRC + [os.path.join(HERE, "csrc", f) for f in (
+    "device.hpp", "kernels.hpp", "host.hpp", "layout.hpp", "encode_kernels.hip",
+    "decode_kernels.hip", "files.inc", "decode.inc", "api.inc")] + [os.path.join(ROOT, "include", "idx2g.h")]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # FP64 lifting must follow the reference's operation order: no contraction
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-I" + os.path.join(ROOT, "include"),
+]
+
+
+def build(force=True, verbose=False):
+    os.makedirs(OUT_DIR, exist_ok=True)
+    if not force and os.path.exists(LIB):
+        t = os.path.getmtime(LIB)
+        if all(os.path.getmtime(d) <= t for d in DEPS):
+            return LIB
+    objs = []
+    for src in SRC:
+        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+        cmd = [HIPCC, *FLAGS, "-c", "-x", "hip" if src.endswith(".hip") else "c++", src, "-o", obj]
+        if src.endswith(".cpp"):
+            cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-ldl"], check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,149 @@
+// Shared host helpers: error reporting, constant tables, zstd binding.
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "host.hpp"
+
+namespace idx2g {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what, int line) {
+  set_error("HIP error %d (%s) in %s at line %d", (int)e, hipGetErrorString(e), what, line);
+  return IDX2G_DEVICE_ERROR;
+}
+
+BrickConst make_brick_const(const Layout& L) {
+  BrickConst k;
+  int mx = 1;
+  for (int d = 0; d < 3; ++d) {
+    k.N[d] = L.ext[d];
+    k.M[d] = L.ext[d] == 1 ? 1 : L.ext[d] - 1;
+    mx = std::max(mx, L.ext[d]);
+  }
+  k.nsb = L.nsb;
+  for (int s = 0; s < 8; ++s)
+    for (int d = 0; d < 3; ++d) {
+      k.sb[s].f[d] = s < L.nsb ? L.sb[s].from[d] : 0;
+      k.sb[s].d[d] = s < L.nsb ? L.sb[s].dims[d] : 1;
+      k.sb[s].s[d] = s < L.nsb ? L.sb[s].strd[d] : 1;
+    }
+  k.ext_levels = msb64((uint64_t)mx);
+  k.order = L.tform_order;
+  k.dtype = L.dtype;
+  k.endbp = L.dtype ? 64 : 32;
+  k.nvox = L.ext[0] * L.ext[1] * L.ext[2];
+  return k;
+}
+
+// Coefficient order of a 4x4x4 block: by i+j+k, then i^2+j^2+k^2, ties as
+// in zfp's published perm_3 table (Source/Core/Zfp.h:414-493); 4x4 order of
+// Zfp.h:497-521.  Entries (i,j,k) with i along x.
+static const signed char kPerm3[64][3] = {
+    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2},
+    {1, 1, 1}, {2, 1, 0}, {2, 0, 1}, {0, 2, 1}, {1, 2, 0}, {1, 0, 2}, {0, 1, 2}, {3, 0, 0}, {0, 3, 0}, {0, 0, 3},
+    {2, 1, 1}, {1, 2, 1}, {1, 1, 2}, {0, 2, 2}, {2, 0, 2}, {2, 2, 0}, {3, 1, 0}, {3, 0, 1}, {0, 3, 1}, {1, 3, 0},
+    {1, 0, 3}, {0, 1, 3}, {1, 2, 2}, {2, 1, 2}, {2, 2, 1}, {3, 1, 1}, {1, 3, 1}, {1, 1, 3}, {3, 2, 0}, {3, 0, 2},
+    {0, 3, 2}, {2, 3, 0}, {2, 0, 3}, {0, 2, 3}, {2, 2, 2}, {3, 2, 1}, {3, 1, 2}, {1, 3, 2}, {2, 3, 1}, {2, 1, 3},
+    {1, 2, 3}, {0, 3, 3}, {3, 0, 3}, {3, 3, 0}, {3, 2, 2}, {2, 3, 2}, {2, 2, 3}, {1, 3, 3}, {3, 1, 3}, {3, 3, 1},
+    {2, 3, 3}, {3, 2, 3}, {3, 3, 2}, {3, 3, 3}};
+static const signed char kPerm2[16][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}, {2, 0}, {0, 2}, {2, 1}, {1, 2},
+                                          {3, 0}, {0, 3}, {2, 2}, {3, 1}, {1, 3}, {3, 2}, {2, 3}, {3, 3}};
+
+PermTables make_perm_tables() {
+  PermTables t;
+  for (int i = 0; i < 64; ++i) {
+    int j = kPerm3[i][0] + 4 * kPerm3[i][1] + 16 * kPerm3[i][2];
+    t.p3[i] = (unsigned char)j;
+    t.ip3[j] = (unsigned char)i;
+  }
+  for (int i = 0; i < 16; ++i) {
+    int j = kPerm2[i][0] + 4 * kPerm2[i][1];
+    t.p2[i] = (unsigned char)j;
+    t.ip2[j] = (unsigned char)i;
+  }
+  return t;
+}
+
+int upload_block_tables(idx2g_dataset* ds) {
+  const Layout& L = ds->L;
+  std::vector<uint32_t> all;
+  for (int l = 0; l < L.nlevels; ++l)
+    for (int s = 0; s < L.nsb; ++s) {
+      ds->blk_off[l][s] = (int)all.size();
+      int ci = 0;
+      for (const BlockInfo& b : L.blocks[l][s]) {
+        all.push_back(pack_block(b.org, b.dims, b.skip, b.skip ? 0 : ci));
+        if (!b.skip) ++ci;
+      }
+    }
+  HIPCHK(hipMalloc(&ds->d_blocks, all.size() * 4 + 4));
+  HIPCHK(hipMemcpy(ds->d_blocks, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  return IDX2G_OK;
+}
+
+// --- zstd (libzstd.so.1, public API of zstd.h) ---
+typedef size_t (*zbound_t)(size_t);
+typedef size_t (*zcomp_t)(void*, size_t, const void*, size_t, int);
+typedef unsigned long long (*zsize_t)(const void*, size_t);
+typedef size_t (*zdec_t)(void*, size_t, const void*, size_t);
+typedef unsigned (*zerr_t)(size_t);
+static zbound_t z_bound;
+static zcomp_t z_comp;
+static zsize_t z_size;
+static zdec_t z_dec;
+static zerr_t z_err;
+static std::once_flag z_once;
+static bool z_ok = false;
+
+static void z_load() {
+  void* h = dlopen("libzstd.so.1", RTLD_NOW | RTLD_LOCAL);
+  if (!h) return;
+  z_bound = (zbound_t)dlsym(h, "ZSTD_compressBound");
+  z_comp = (zcomp_t)dlsym(h, "ZSTD_compress");
+  z_size = (zsize_t)dlsym(h, "ZSTD_getFrameContentSize");
+  z_dec = (zdec_t)dlsym(h, "ZSTD_decompress");
+  z_err = (zerr_t)dlsym(h, "ZSTD_isError");
+  z_ok = z_bound && z_comp && z_size && z_dec && z_err;
+}
+
+// CompressBufZstd, Source/Core/idx2Encode.cpp:23-37 (level 1)
+int zstd_compress(const void* src, size_t n, std::vector<uint8_t>* out) {
+  std::call_once(z_once, z_load);
+  out->clear();
+  if (!z_ok) { set_error("libzstd.so.1 not available"); return IDX2G_NOT_SUPPORTED; }
+  if (n == 0) return IDX2G_OK;
+  out->resize(z_bound(n));
+  size_t c = z_comp(out->data(), out->size(), src, n, 1);
+  if (z_err(c)) { set_error("zstd compression failed"); return IDX2G_UNKNOWN; }
+  out->resize(c);
+  return IDX2G_OK;
+}
+
+// DecompressBufZstd, Source/Core/idx2Decode.cpp:53-78
+int zstd_decompress(const void* src, size_t n, std::vector<uint8_t>* out) {
+  std::call_once(z_once, z_load);
+  if (!z_ok) { set_error("libzstd.so.1 not available"); return IDX2G_NOT_SUPPORTED; }
+  unsigned long long sz = z_size(src, n);
+  if (sz == (unsigned long long)-1 || sz == (unsigned long long)-2) { set_error("bad zstd frame"); return IDX2G_PARSE_FAILED; }
+  out->resize(sz);
+  size_t r = z_dec(out->data(), sz, src, n);
+  if (z_err(r) || r != sz) { set_error("zstd decompression failed"); return IDX2G_PARSE_FAILED; }
+  return IDX2G_OK;
+}
+
+}  // namespace idx2g
+
+extern "C" const char* idx2g_last_error_message(void) { return idx2g::g_err; }

@@ -1,0 +1,254 @@
+// Decode kernels for gfx950: one workgroup per brick runs DecodeBrick
+// (Source/Core/idx2Decode.cpp:238-321): parent subband-0 copy, per-subband
+// bit-plane decoding (DecodeSubband, idx2Decode.cpp:85-235), inverse zfp and
+// dequantisation, InverseCdf53, and the copy into the output grid
+// (idx2Decode.cpp:431-445).
+//
+// Bit-plane streams of one brick-subband are walked by one wave: lane j owns
+// the key (chunk channel) khi - j and decodes that plane of block t - j at
+// step t, receiving the block's significance count N from lane j-1 (the next
+// higher plane) through a lane shuffle.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace idx2g {
+
+// 64 bits of the blob starting at bit `pos` (blob padded by >= 16 bytes)
+__device__ __forceinline__ u64 peek64(const uint8_t* blob, u64 pos) {
+  const uint32_t* w = (const uint32_t*)blob + (pos >> 5);
+  const int sh = int(pos & 31);
+  const u64 lo = (u64)w[0] | ((u64)w[1] << 32);
+  if (!sh) return lo;
+  return (lo >> sh) | ((u64)w[2] << (64 - sh));
+}
+
+// One bit plane (Decode, Source/Core/Zfp.h:988-1059): returns the plane word.
+__device__ __forceinline__ u64 decode_plane(const uint8_t* blob, u64& pos, int& n, int nv) {
+  u64 X = 0;
+  if (n > 0) {
+    const u64 w = peek64(blob, pos);
+    X = n >= 64 ? w : (w & ((1ull << n) - 1));
+    pos += n;
+  }
+  while (n < nv) {
+    u64 w = peek64(blob, pos);
+    if (!(w & 1)) { pos += 1; break; }   // group insignificant
+    pos += 1;
+    w >>= 1;
+    const int lim = nv - 1 - n;          // bits examined before the implicit last one
+    const int z = w ? __ffsll((long long)w) - 1 : 64;
+    if (z < lim) { n += z; pos += z + 1; }
+    else { pos += lim; n = nv - 1; }
+    X |= 1ull << n;
+    ++n;
+  }
+  return X;
+}
+
+struct DecLds {
+  u64* pw;              // [kMaxBlocks][npl]
+  short* emax;          // [kMaxBlocks]
+  unsigned char* nbps;  // [kMaxBlocks] planes decoded
+  unsigned char* nsave; // [kMaxBlocks] N after the last key group
+  unsigned char* ssave; // [kMaxBlocks] stopped flag after the last key group
+};
+
+__host__ __device__ inline size_t dec_lds_bytes(int npl) {
+  return size_t(kMaxBlocks) * npl * 8 + kMaxBlocks * (2 + 3) + 64;
+}
+
+__device__ inline DecLds dcarve(unsigned char* base, int npl) {
+  DecLds s;
+  size_t o = 0;
+  s.pw = (u64*)(base + o); o += size_t(kMaxBlocks) * npl * 8;
+  s.emax = (short*)(base + o); o += kMaxBlocks * 2;
+  s.nbps = base + o; o += kMaxBlocks;
+  s.nsave = base + o; o += kMaxBlocks;
+  s.ssave = base + o;
+  return s;
+}
+
+__device__ inline void dunpack(uint32_t w, int org[3], int dims[3], int* skip, int* ci) {
+  org[0] = w & 31; org[1] = (w >> 5) & 31; org[2] = (w >> 10) & 31;
+  dims[0] = (w >> 15) & 7; dims[1] = (w >> 18) & 7; dims[2] = (w >> 21) & 7;
+  *skip = (w >> 24) & 1;
+  *ci = int(w >> 25);
+}
+
+__global__ void __launch_bounds__(kThreads) k_decode_level(DecArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ int s_klo, s_khi;
+  const BrickConst& k = a.k;
+  const int npl = k.dtype ? 64 : 32;
+  DecLds S = dcarve(lds_raw, npl);
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
+  const int prec3[4] = {63, 62, 61, 60};
+
+  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+    double* F = a.bricks + (size_t)bi * k.nvox;
+    const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
+    const u64 lin = a.lin[bi];
+    const int dense = a.dense[bi];
+    for (int i = tid; i < k.nvox; i += blockDim.x) F[i] = 0;
+    __syncthreads();
+    for (int sb = 0; sb < k.nsb; ++sb) {
+      if (!((a.mask >> sb) & 1)) continue;
+      const DGrid& g = k.sb[sb];
+      // parent's decoded interior into subband 0 (CopyExtentGrid, Volume.h:1153-1173)
+      if (sb == 0 && !a.coarsest) {
+        const double* P = a.parents + (size_t)a.parent_slot[bi] * k.nvox;
+        int lp[3];
+        for (int d = 0; d < 3; ++d) lp[d] = a.brick[d] > 1 ? (b3[d] & 1) : 0;
+        const int n = a.sbne[0] * a.sbne[1] * a.sbne[2];
+        for (int i = tid; i < n; i += blockDim.x) {
+          int x = i % a.sbne[0], y = (i / a.sbne[0]) % a.sbne[1], z = i / (a.sbne[0] * a.sbne[1]);
+          F[vox(k, g.f[0] + g.s[0] * x, g.f[1] + g.s[1] * y, g.f[2] + g.s[2] * z)] =
+              P[vox(k, lp[0] * a.sbne[0] + x, lp[1] * a.sbne[1] + y, lp[2] * a.sbne[2] + z)];
+        }
+      }
+      const int nblk = a.blk_n[sb];
+      const uint32_t* blocks = a.blocks + a.blk_off[sb];
+      if (tid == 0) { s_klo = 0x7fffffff; s_khi = -0x7fffffff; }
+      __syncthreads();
+      // exponents of the coded blocks and the key range to walk
+      for (int b = tid; b < nblk; b += blockDim.x) {
+        int org[3], bd[3], skip, ci;
+        dunpack(blocks[b], org, bd, &skip, &ci);
+        S.nbps[b] = 0;
+        S.nsave[b] = 0;
+        S.ssave[b] = 0;
+        for (int p = 0; p < npl; ++p) S.pw[b * npl + p] = 0;
+        if (skip) { S.emax[b] = kSkip; continue; }
+        const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
+        const int e = k.dtype ? int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023
+                              : int(a.exps[row]) - 127;
+        S.emax[b] = (short)e;
+        const int stop = max(64 - k.endbp, 57 + a.etol - e);
+        if (stop <= 63) {
+          atomicMax(&s_khi, 63 + e + kBpBiasDev);
+          atomicMin(&s_klo, stop + e + kBpBiasDev);
+        }
+      }
+      __syncthreads();
+      const int klo = s_klo, khi = s_khi;
+      // ---- pipelined stream walk (wave 0), key groups of 64 from the top ----
+      if (wave == 0 && khi >= klo) {
+        for (int g0 = khi; g0 >= klo; g0 -= 64) {
+          const int key = g0 - lane;
+          const bool kact = key >= klo;
+          u64 pos = 0;
+          bool have = false;
+          if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
+            const u64 d = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
+            if (d) { pos = (d - 1) * 8; have = true; }
+          }
+          int n_out = 0;
+          bool stop_out = false;
+          for (int t = 0; t < nblk + 64; ++t) {
+            const int n_left = __shfl_up(n_out, 1, 64);
+            const int s_left = __shfl_up((int)stop_out, 1, 64);
+            const int b = t - lane;
+            const bool inb = b >= 0 && b < nblk;
+            const bool valid = kact && inb && S.emax[inb ? b : 0] != kSkip;
+            if (valid) {
+              int n_in = lane == 0 ? (g0 == khi ? 0 : S.nsave[b]) : n_left;
+              bool st_in = lane == 0 ? (g0 == khi ? false : S.ssave[b] != 0) : (s_left != 0);
+              const int e = S.emax[b];
+              const int bp = key - kBpBiasDev - e;
+              const int stop = max(64 - k.endbp, 57 + a.etol - e);
+              n_out = n_in;
+              stop_out = st_in;
+              if (bp <= 63 && bp >= stop && !st_in) {
+                if (!have) {
+                  stop_out = true;  // missing chunk: the block stops decoding planes
+                } else {
+                  int org[3], bd[3], skip, ci;
+                  dunpack(blocks[b], org, bd, &skip, &ci);
+                  const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
+                  int n = n_in;
+                  const u64 X = decode_plane(a.blob, pos, n, nv);
+                  n_out = n;
+                  S.pw[b * npl + (63 - bp)] = X;
+                }
+              } else if (bp > 63) {
+                n_out = 0;
+              }
+              if (lane == 63 || key == klo) {
+                S.nsave[b] = (unsigned char)n_out;
+                S.ssave[b] = stop_out ? 1 : 0;
+              }
+            }
+          }
+        }
+      }
+      __syncthreads();
+      // ---- inverse block stage: one wave per block, one coefficient per lane ----
+      for (int b = wave; b < nblk; b += nwaves) {
+        const int e = S.emax[b];
+        if (e == kSkip) continue;
+        int org[3], bd[3], skip, ci;
+        dunpack(blocks[b], org, bd, &skip, &ci);
+        bool bypass = false;
+        for (int d = 0; d < 3; ++d) bypass |= (org[d] % a.spacing[sb][d]) != 0;
+        const int stop = max(64 - k.endbp, 57 + a.etol - e);
+        const int np = stop > 63 ? 0 : 64 - stop;
+        if (np == 0 || bypass) continue;
+        const int nd = (bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1);
+        const int nv = 1 << (2 * nd);
+        u64 u = 0;
+        for (int p = 0; p < np; ++p) u |= ((S.pw[b * npl + p] >> lane) & 1ull) << (63 - p);
+        // InverseShuffle (Zfp.h:580-605): coefficient j comes from position ip[j]
+        int src = nd == 3 ? a.perm.ip3[lane] : nd == 2 ? a.perm.ip2[lane & 15] : lane;
+        const u64 us = (u64)shfl_i64((i64)u, src);
+        i64 q = (i64)((us ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+        q = zfp_inverse_lanes(q, lane, nd);
+        // Dequantize (Function.h:139-149)
+        const double scale = 1.0 / pow2d(prec3[nd] - 1 - e);
+        const double f = scale * i64_to_f64(q);
+        if (lane < nv) {
+          int x = lane % bd[0], y = (lane / bd[0]) % bd[1], z = lane / (bd[0] * bd[1]);
+          F[vox(k, g.f[0] + g.s[0] * (org[0] + x), g.f[1] + g.s[1] * (org[1] + y), g.f[2] + g.s[2] * (org[2] + z))] = f;
+        }
+      }
+      __syncthreads();
+    }
+    inverse_brick(k, F, a.winv, a.coarsest);
+    // ---- copy into the output grid: Crop(OutGrid, BrickGrid) ----
+    if (a.out) {
+      int cf[3], cd[3], bf[3];
+      bool any = true;
+      for (int d = 0; d < 3; ++d) {
+        bf[d] = b3[d] * a.B3[d];
+        const int blast = bf[d] + (a.brick[d] - 1) * a.bstride;
+        const int olast = a.og_f[d] + (a.og_d[d] - 1) * a.og_s[d];
+        int frst = max(a.og_f[d], bf[d]);
+        int last = min(olast, blast);
+        frst = ((frst - a.og_f[d] + a.og_s[d] - 1) / a.og_s[d]) * a.og_s[d] + a.og_f[d];
+        last = ((last - a.og_f[d]) / a.og_s[d]) * a.og_s[d] + a.og_f[d];
+        cf[d] = frst;
+        cd[d] = frst <= last ? (last - frst) / a.og_s[d] + 1 : 0;
+        any = any && cd[d] > 0;
+      }
+      if (any) {
+        const int n = cd[0] * cd[1] * cd[2];
+        for (int i = tid; i < n; i += blockDim.x) {
+          const int ix = i % cd[0], iy = (i / cd[0]) % cd[1], iz = i / (cd[0] * cd[1]);
+          const int p[3] = {cf[0] + ix * a.og_s[0], cf[1] + iy * a.og_s[1], cf[2] + iz * a.og_s[2]};
+          int s3[3], o3[3];
+          for (int d = 0; d < 3; ++d) {
+            s3[d] = (p[d] - bf[d]) / a.bstride;
+            o3[d] = (p[d] - a.og_f[d]) / a.og_s[d];
+          }
+          const double v = F[vox(k, s3[0], s3[1], s3[2])];
+          const size_t oi = ((size_t)o3[2] * a.og_d[1] + o3[1]) * (size_t)a.og_d[0] + o3[0];
+          if (k.dtype) ((double*)a.out)[oi] = v;
+          else ((float*)a.out)[oi] = (float)v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace idx2g

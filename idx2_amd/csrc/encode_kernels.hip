@@ -1,0 +1,524 @@
+// Encode kernels for gfx950: one workgroup per brick runs the whole per-brick
+// path of EncodeBrick (Source/Core/idx2Encode.cpp:276-340): gather,
+// ExtrapolateCdf53, one-level CDF5/3 with subband weights, subband-0 copy to
+// the parent, then per subband the zfp block stage and the embedded coder
+// (idx2Encode.cpp:141-272).  Streams of one brick-subband are assembled in
+// LDS and appended to a device arena; chunk assembly (idx2Write.cpp:207-290)
+// runs in the kernels at the end of this file.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace idx2g {
+
+struct EncShared {
+  int klo, khi, err;
+  double wmn[kThreads / 64], wmx[kThreads / 64];
+};
+
+// LDS layout for one subband (dynamic shared memory), npl planes per block
+struct EncLds {
+  u64* pw;          // [kMaxBlocks][npl] plane words
+  uint32_t* poff;   // [kMaxBlocks][npl] bit offset inside the key stream
+  uint16_t* plen;   // [kMaxBlocks][npl] emitted bits
+  short* emax;      // [kMaxBlocks]
+  unsigned char* np;// [kMaxBlocks]
+  uint32_t* kbits;  // [kMaxKeys]
+  uint32_t* kboff;  // [kMaxKeys] byte offset of the key stream in the subband
+  short* kfirst;    // [kMaxKeys]
+  uint32_t* stage;  // [kStageWords]
+};
+
+__host__ __device__ inline size_t enc_lds_bytes(int npl) {
+  return size_t(kMaxBlocks) * npl * (8 + 4 + 2) + kMaxBlocks * 3 + kMaxKeys * (4 + 4 + 2) + 64 +
+         size_t(kStageWords) * 4 + 256;
+}
+
+__device__ inline EncLds carve(unsigned char* base, int npl) {
+  EncLds s;
+  size_t o = 0;
+  s.pw = (u64*)(base + o); o += size_t(kMaxBlocks) * npl * 8;
+  s.stage = (uint32_t*)(base + o); o += size_t(kStageWords) * 4;
+  s.poff = (uint32_t*)(base + o); o += size_t(kMaxBlocks) * npl * 4;
+  s.kbits = (uint32_t*)(base + o); o += kMaxKeys * 4;
+  s.kboff = (uint32_t*)(base + o); o += kMaxKeys * 4;
+  s.plen = (uint16_t*)(base + o); o += size_t(kMaxBlocks) * npl * 2;
+  s.kfirst = (short*)(base + o); o += kMaxKeys * 2;
+  s.emax = (short*)(base + o); o += kMaxBlocks * 2;
+  s.np = (unsigned char*)(base + o);
+  return s;
+}
+
+__device__ inline void unpack_block(uint32_t w, int org[3], int dims[3], int* skip, int* ci) {
+  org[0] = w & 31; org[1] = (w >> 5) & 31; org[2] = (w >> 10) & 31;
+  dims[0] = (w >> 15) & 7; dims[1] = (w >> 18) & 7; dims[2] = (w >> 21) & 7;
+  *skip = (w >> 24) & 1;
+  *ci = int(w >> 25);
+}
+
+__device__ inline void flag_error(unsigned long long* counters, unsigned long long bit) {
+  atomicOr(&counters[2], bit);
+}
+
+// OR `len` bits of w (LSB first) at bit position pos of a 32-bit word array
+template <bool kShared>
+__device__ inline void or_bits(uint32_t* words, u64 pos, const u64 w[3], int len) {
+  if (len <= 0) return;
+  const int sh = int(pos & 31);
+  u64 wi = pos >> 5;
+  // shifted view of w: 32-bit pieces
+  const int nw = (sh + len + 31) >> 5;
+  for (int i = 0; i < nw; ++i) {
+    // bits [32*i - sh, 32*i - sh + 32) of w
+    int b0 = 32 * i - sh;
+    u64 v;
+    if (b0 >= 0) {
+      int q = b0 >> 6, r = b0 & 63;
+      v = w[q] >> r;
+      if (r > 32 && q + 1 < 3) v |= w[q + 1] << (64 - r);
+    } else {
+      v = w[0] << (-b0);
+    }
+    uint32_t piece = uint32_t(v);
+    if (piece) {
+      if (kShared) atomicOr(&words[wi + i], piece);
+      else atomicOr(&words[wi + i], piece);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_encode_level(EncArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ EncShared sh;
+  const BrickConst& k = a.k;
+  const int npl = k.dtype ? 64 : 32;
+  EncLds S = carve(lds_raw, npl);
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
+  const int prec3[4] = {63, 62, 61, 60};
+
+  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+    const int dense = a.b0 + bi;
+    const int b3[3] = {a.xyz[3 * dense], a.xyz[3 * dense + 1], a.xyz[3 * dense + 2]};
+    double* F;
+    int ext[3];
+    if (a.level == 0) {
+      // gather the brick into a zeroed extended volume (brick_copier::Copy,
+      // Source/Core/idx2Encode.cpp:377-386, Volume.h:1123-1149)
+      F = a.scratch + (size_t)blockIdx.x * k.nvox;
+      int from[3];
+      for (int d = 0; d < 3; ++d) {
+        from[d] = b3[d] * a.brick[d];
+        ext[d] = max(0, min(a.brick[d], a.dims[d] - from[d]));
+      }
+      double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+      for (int i = tid; i < k.nvox; i += blockDim.x) {
+        int x = i % k.N[0], y = (i / k.N[0]) % k.N[1], z = i / (k.N[0] * k.N[1]);
+        double v = 0;
+        if (x < ext[0] && y < ext[1] && z < ext[2]) {
+          size_t src = ((size_t)(from[2] + z) * a.dims[1] + (from[1] + y)) * (size_t)a.dims[0] + (from[0] + x);
+          v = k.dtype ? ((const double*)a.input)[src] : (double)((const float*)a.input)[src];
+          mn = v < mn ? v : mn;
+          mx = v > mx ? v : mx;
+        }
+        F[i] = v;
+      }
+      mn = -wave_max(-mn);
+      mx = wave_max(mx);
+      if (lane == 0) { sh.wmn[wave] = mn; sh.wmx[wave] = mx; }
+      __syncthreads();
+      if (tid == 0) {
+        for (int w = 1; w < nwaves; ++w) {
+          mn = sh.wmn[w] < mn ? sh.wmn[w] : mn;
+          mx = sh.wmx[w] > mx ? sh.wmx[w] : mx;
+        }
+        a.minmax[2 * dense] = mn;
+        a.minmax[2 * dense + 1] = mx;
+      }
+    } else {
+      F = a.bricks + (size_t)dense * k.nvox;
+      for (int d = 0; d < 3; ++d) {
+        if (a.brick[d] > 1) {
+          int nch = min(2, a.nbricks_prev[d] - 2 * b3[d]);
+          ext[d] = nch * a.sbne[d];
+        } else {
+          ext[d] = 1;
+        }
+      }
+    }
+    __syncthreads();
+    extrapolate(k, F, ext);
+    forward_brick(k, F, a.wfwd, a.coarsest);
+
+    // subband 0 (non-extended interior) into the parent brick
+    // (EncodeBrick, idx2Encode.cpp:303-333; CopyGridExtent, Volume.h:1177-1196)
+    if (!a.coarsest) {
+      double* P = a.parents + (size_t)a.parent_of[dense] * k.nvox;
+      const DGrid& g = k.sb[0];
+      int lp[3];
+      for (int d = 0; d < 3; ++d) lp[d] = a.brick[d] > 1 ? (b3[d] & 1) : 0;
+      const int n = a.sbne[0] * a.sbne[1] * a.sbne[2];
+      for (int i = tid; i < n; i += blockDim.x) {
+        int x = i % a.sbne[0], y = (i / a.sbne[0]) % a.sbne[1], z = i / (a.sbne[0] * a.sbne[1]);
+        P[vox(k, lp[0] * a.sbne[0] + x, lp[1] * a.sbne[1] + y, lp[2] * a.sbne[2] + z)] =
+            F[vox(k, g.f[0] + g.s[0] * x, g.f[1] + g.s[1] * y, g.f[2] + g.s[2] * z)];
+      }
+    }
+
+    for (int sb = 0; sb < k.nsb; ++sb) {
+      const DGrid& g = k.sb[sb];
+      const int nblk = a.blk_n[sb];
+      const uint32_t* blocks = a.blocks + a.blk_off[sb];
+      if (tid == 0) { sh.klo = 0x7fffffff; sh.khi = -0x7fffffff; sh.err = 0; }
+      __syncthreads();
+      // ---- stage A: one wave per block, one coefficient per lane ----
+      for (int b = wave; b < nblk; b += nwaves) {
+        int org[3], bd[3], skip, ci;
+        unpack_block(blocks[b], org, bd, &skip, &ci);
+        if (skip) {
+          if (lane == 0) { S.emax[b] = kSkip; S.np[b] = 0; }
+          continue;
+        }
+        const int nd = (bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1);
+        const int nv = 1 << (2 * nd);
+        const bool act = lane < nv;
+        // gather order: X fastest over the block dims (idx2_BeginFor3)
+        int x = lane % bd[0], y = (lane / bd[0]) % bd[1], z = lane / (bd[0] * bd[1]);
+        double v = 0;
+        if (act)
+          v = F[vox(k, g.f[0] + g.s[0] * (org[0] + x), g.f[1] + g.s[1] * (org[1] + y),
+                    g.f[2] + g.s[2] * (org[2] + z))];
+        // QuantizeF32/F64 (Source/Core/Function.h:58-89)
+        const double maxabs = wave_max(act ? fabs(v) : 0.0);
+        const int emax = k.dtype ? exponent_f64(maxabs) : exponent_f32((float)maxabs);
+        const int prec = prec3[nd];
+        const double scale = pow2d(prec - 1 - emax);
+        i64 q = act ? f64_to_i64(scale * v) : 0;
+        q = zfp_forward_lanes(q, lane, nd);
+        // ForwardShuffle + negabinary (Zfp.h:534-559)
+        int src = nd == 3 ? a.perm.p3[lane] : nd == 2 ? a.perm.p2[lane & 15] : lane;
+        i64 qq = shfl_i64(q, src);
+        const u64 u = ((u64)qq + IDX2G_NBMASK) ^ IDX2G_NBMASK;
+        int stop = max(64 - k.endbp, 57 + a.etol - emax);
+        int np = stop > 63 ? 0 : 64 - stop;
+        for (int p = 0; p < np; ++p) {
+          u64 X = __ballot(act && ((u >> (63 - p)) & 1ull));
+          if (lane == 0) S.pw[b * npl + p] = X;
+        }
+        if (lane == 0) {
+          S.emax[b] = (short)emax;
+          S.np[b] = (unsigned char)np;
+          if (np > 0) {
+            atomicMax(&sh.khi, 63 + emax + kBpBiasDev);
+            atomicMin(&sh.klo, stop + emax + kBpBiasDev);
+          }
+          // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
+          const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
+          if (k.dtype) {
+            uint16_t e = (uint16_t)(emax + 1023);
+            a.exps[row] = (uint8_t)(e & 0xff);
+            a.exps[row + 1] = (uint8_t)(e >> 8);
+          } else {
+            a.exps[row] = (uint8_t)(emax + 127);
+          }
+        }
+      }
+      __syncthreads();
+      // ---- stage B: emitted length of every (block, plane), thread per block ----
+      for (int b = tid; b < nblk; b += blockDim.x) {
+        if (S.emax[b] == kSkip) continue;
+        int org[3], bd[3], skip, ci;
+        unpack_block(blocks[b], org, bd, &skip, &ci);
+        const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
+        int n = 0;
+        for (int p = 0; p < S.np[b]; ++p) S.plen[b * npl + p] = (uint16_t)plane_len(S.pw[b * npl + p], n, nv);
+      }
+      __syncthreads();
+      const int klo = sh.klo, khi = sh.khi;
+      const int nk = khi >= klo ? khi - klo + 1 : 0;
+      if (nk > kMaxKeys) {
+        if (tid == 0) { flag_error(a.counters, 1); a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0}; }
+        __syncthreads();
+        continue;
+      }
+      // ---- stage C: per key, bit offsets of each block's plane (Morton order) ----
+      for (int kk = tid; kk < nk; kk += blockDim.x) {
+        const int key = klo + kk;
+        uint32_t bits = 0;
+        int first = -1;
+        for (int b = 0; b < nblk; ++b) {
+          const int e = S.emax[b];
+          if (e == kSkip) continue;
+          const int p = 63 - (key - kBpBiasDev - e);
+          if (p >= 0 && p < S.np[b]) {
+            S.poff[b * npl + p] = bits;
+            bits += S.plen[b * npl + p];
+            if (first < 0) first = b;
+          }
+        }
+        S.kbits[kk] = bits;
+        S.kfirst[kk] = (short)first;
+      }
+      __syncthreads();
+      // ---- stage D: byte layout of the key streams, arena + entry reservation ----
+      __shared__ unsigned long long s_base, s_ebase;
+      __shared__ uint32_t s_total;
+      if (tid == 0) {
+        uint32_t tot = 0;
+        for (int kk = 0; kk < nk; ++kk) {
+          S.kboff[kk] = tot;
+          tot += (S.kbits[kk] + 7) >> 3;
+        }
+        s_total = tot;
+        const unsigned long long alloc = (tot + 3ull) & ~3ull;
+        s_base = alloc ? atomicAdd(&a.counters[0], alloc) : 0;
+        s_ebase = nk ? atomicAdd(&a.counters[1], (unsigned long long)nk) : 0;
+        const bool over = s_base + alloc > a.arena_cap || s_ebase + nk > a.entry_cap;
+        if (over) flag_error(a.counters, 2);
+        SbHdr h;
+        h.klo = klo;
+        h.nkeys = over ? 0 : nk;
+        h.ebase = s_ebase;
+        a.hdrs[(size_t)dense * k.nsb + sb] = h;
+      }
+      __syncthreads();
+      const unsigned long long base = s_base;
+      const uint32_t total = s_total;
+      if (base + ((total + 3ull) & ~3ull) > a.arena_cap || s_ebase + nk > a.entry_cap) continue;
+      for (int kk = tid; kk < nk; kk += blockDim.x) {
+        SbEntry e;
+        e.off = base + S.kboff[kk];
+        e.size = (S.kbits[kk] + 7) >> 3;
+        e.first_block = (uint16_t)(S.kfirst[kk] < 0 ? 0xffff : S.kfirst[kk]);
+        e.key = (uint16_t)(klo + kk);
+        a.entries[s_ebase + kk] = e;
+      }
+      // ---- stage E: write the streams (LDS staging, else global) ----
+      const uint32_t nwords = (total + 3) >> 2;
+      const bool staged = nwords <= (uint32_t)kStageWords;
+      uint32_t* gwords = (uint32_t*)(a.arena + base);
+      if (staged) {
+        for (uint32_t i = tid; i < nwords; i += blockDim.x) S.stage[i] = 0;
+      } else {
+        for (uint32_t i = tid; i < nwords; i += blockDim.x) gwords[i] = 0;
+        __threadfence();
+      }
+      __syncthreads();
+      for (int b = tid; b < nblk; b += blockDim.x) {
+        const int e = S.emax[b];
+        if (e == kSkip) continue;
+        int org[3], bd[3], skip, ci;
+        unpack_block(blocks[b], org, bd, &skip, &ci);
+        const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
+        int n = 0;
+        for (int p = 0; p < S.np[b]; ++p) {
+          const u64 X = S.pw[b * npl + p];
+          u64 w[3];
+          const int len = plane_emit(X, n, nv, w);
+          plane_len(X, n, nv);  // advance n
+          const int kk = (63 - p) + e + kBpBiasDev - klo;
+          const u64 pos = (u64)S.kboff[kk] * 8 + S.poff[b * npl + p];
+          if (staged) or_bits<true>(S.stage, pos, w, len);
+          else or_bits<false>(gwords, pos, w, len);
+        }
+      }
+      __syncthreads();
+      if (staged)
+        for (uint32_t i = tid; i < nwords; i += blockDim.x) gwords[i] = S.stage[i];
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
+// chunk assembly (WriteChunk, Source/Core/idx2Write.cpp:207-290):
+//   varbyte(NBricks) | varbyte(first brick) unary(gaps) | varbyte(sizes) | payloads
+// The unary gap codes of consecutive bricks form a presence bitmap: brick i
+// sets bit (brick_i - first - 1) after the first brick's varbyte.
+// --------------------------------------------------------------------------
+__device__ inline int varbyte_len(u64 v) {
+  int n = 1;
+  while (v >>= 7) ++n;
+  return n;
+}
+
+// Key range of a level from the per-brick-subband headers.
+__global__ void k_key_range(const SbHdr* hdrs, int n, int* out) {
+  int lo = 0x7fffffff, hi = -0x7fffffff;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (hdrs[i].nkeys > 0) {
+      lo = min(lo, hdrs[i].klo);
+      hi = max(hi, hdrs[i].klo + hdrs[i].nkeys - 1);
+    }
+  }
+  atomicMin(&out[0], lo);
+  atomicMax(&out[1], hi);
+}
+
+// dense index table: idx[(sb*nk + key-klo)*span + lin] = entry + 1
+__global__ void k_fill_index(const SbHdr* hdrs, const u64* lin, int nb, int nsb, int klo, int nk, u64 span,
+                             uint32_t* idx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb * nsb) return;
+  const int dense = i / nsb, sb = i % nsb;
+  const SbHdr h = hdrs[i];
+  for (int j = 0; j < h.nkeys; ++j) {
+    const int kk = h.klo + j - klo;
+    idx[((size_t)sb * nk + kk) * span + lin[dense]] = (uint32_t)(h.ebase + j + 1);
+  }
+}
+
+// One workgroup per (subband, key, chunk): sizes of the chunk.
+__global__ void __launch_bounds__(256) k_chunk_size(const uint32_t* idx, const SbEntry* entries, int nk, u64 span,
+                                                    int bpc, int nchunks, ChunkRec* recs, int klo) {
+  const int r = blockIdx.x;
+  const int chunk = r % nchunks, kk = (r / nchunks) % nk, sb = r / (nchunks * nk);
+  const u64 b0 = (u64)chunk * bpc, b1 = min(b0 + (u64)bpc, span);
+  const uint32_t* row = idx + ((size_t)sb * nk + kk) * span;
+  unsigned long long cnt = 0, sz = 0, vb = 0;
+  u64 first = ~0ull, last = 0;
+  for (u64 b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    uint32_t e = row[b];
+    if (!e || entries[e - 1].size == 0) continue;
+    const uint32_t s = entries[e - 1].size;
+    cnt += 1;
+    sz += s;
+    vb += varbyte_len(s);
+    first = min(first, b);
+    last = max(last, b);
+  }
+  __shared__ unsigned long long s_cnt, s_sz, s_vb, s_first, s_last;
+  if (threadIdx.x == 0) { s_cnt = s_sz = s_vb = 0; s_first = ~0ull; s_last = 0; }
+  __syncthreads();
+  if (cnt) {
+    atomicAdd(&s_cnt, cnt);
+    atomicAdd(&s_sz, sz);
+    atomicAdd(&s_vb, vb);
+    atomicMin(&s_first, first);
+    atomicMax(&s_last, last);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ChunkRec c;
+    c.off = 0;
+    c.nbricks = (uint32_t)s_cnt;
+    c.sb = sb;
+    c.key = klo + kk;
+    c.chunk = chunk;
+    c.first_brick = s_first;
+    c.last_brick = s_last;
+    c.first_fb = 0xffff;
+    c.size = 0;
+    if (s_cnt) {
+      const u64 delta_bits = 8ull * varbyte_len(s_first) + (s_last - s_first);
+      c.size = (uint32_t)(varbyte_len(s_cnt) + (delta_bits + 7) / 8 + s_vb + s_sz);
+      c.first_fb = entries[row[s_first] - 1].first_block;
+    }
+    recs[r] = c;
+  }
+}
+
+// One workgroup per non-empty chunk (in blob order): write the header, the
+// varbyte sizes and the payloads, and record where each brick's stream
+// starts (decode directory: blob offset + 1).
+__global__ void __launch_bounds__(256) k_chunk_write(const uint32_t* idx, const SbEntry* entries, const uint8_t* arena,
+                                                     int nk, int klo, u64 span, int bpc, const ChunkRec* recs,
+                                                     const int* order, uint8_t* blob, u64* dir) {
+  const ChunkRec c = recs[order[blockIdx.x]];
+  const size_t rowi = ((size_t)c.sb * nk + (c.key - klo)) * span;
+  const uint32_t* row = idx + rowi;
+  u64* drow = dir + rowi;
+  const u64 b0 = (u64)c.chunk * bpc, b1 = min(b0 + (u64)bpc, span);
+  uint8_t* out = blob + c.off;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int vn = varbyte_len(c.nbricks), vf = varbyte_len(c.first_brick);
+  const u64 nbm = c.last_brick - c.first_brick;  // bitmap bits after the first varbyte
+  const u64 dbytes = (8ull * vf + nbm + 7) / 8;
+  auto present = [&](u64 b) -> bool {
+    const uint32_t e = row[b];
+    return e && entries[e - 1].size > 0;
+  };
+  if (tid == 0) {
+    u64 v = c.nbricks;
+    int i = 0;
+    do { uint8_t byte = v & 0x7f; v >>= 7; out[i++] = byte | (v ? 0x80 : 0); } while (v);
+    v = c.first_brick;
+    i = 0;
+    do { uint8_t byte = v & 0x7f; v >>= 7; out[vn + i++] = byte | (v ? 0x80 : 0); } while (v);
+  }
+  // unary gaps as a presence bitmap, one byte per thread
+  for (u64 j = (u64)vf + tid; j < dbytes; j += blockDim.x) {
+    uint8_t byte = 0;
+    for (int i = 0; i < 8; ++i) {
+      const u64 bit = 8 * (j - vf) + i;
+      if (bit < nbm && present(c.first_brick + 1 + bit)) byte |= uint8_t(1u << i);
+    }
+    out[vn + j] = byte;
+  }
+  // total varbyte bytes of the sizes section
+  __shared__ unsigned long long s_vbtotal, s_vboff, s_doff;
+  __shared__ uint32_t s1[256], s2[256];
+  __shared__ u64 s_src[256], s_dst[256];
+  __shared__ uint32_t s_len[256];
+  __shared__ int s_n;
+  if (tid == 0) { s_vbtotal = 0; s_vboff = 0; s_doff = 0; }
+  __syncthreads();
+  unsigned long long vbl = 0;
+  for (u64 b = b0 + tid; b < b1; b += blockDim.x) {
+    const uint32_t e = row[b];
+    if (e && entries[e - 1].size) vbl += varbyte_len(entries[e - 1].size);
+  }
+  if (vbl) atomicAdd(&s_vbtotal, vbl);
+  __syncthreads();
+  const u64 sizes_off = (u64)vn + dbytes;
+  const u64 payload_start = sizes_off + s_vbtotal;
+  for (u64 t0 = b0; t0 < b1; t0 += blockDim.x) {
+    const u64 b = t0 + tid;
+    uint32_t sz = 0;
+    u64 src = 0;
+    if (b < b1) {
+      const uint32_t e = row[b];
+      if (e) { sz = entries[e - 1].size; src = entries[e - 1].off; }
+    }
+    const uint32_t v1 = sz ? (uint32_t)varbyte_len(sz) : 0u;
+    s1[tid] = v1;
+    s2[tid] = sz;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      uint32_t a1 = 0, a2 = 0;
+      if (tid >= o) { a1 = s1[tid - o]; a2 = s2[tid - o]; }
+      __syncthreads();
+      s1[tid] += a1;
+      s2[tid] += a2;
+      __syncthreads();
+    }
+    const u64 vboff = s_vboff + s1[tid] - v1;
+    const u64 doff = s_doff + s2[tid] - sz;
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+    if (sz) {
+      uint8_t* vp = out + sizes_off + vboff;
+      u64 v = sz;
+      int i = 0;
+      do { uint8_t byte = v & 0x7f; v >>= 7; vp[i++] = byte | (v ? 0x80 : 0); } while (v);
+      const u64 dst = c.off + payload_start + doff;
+      drow[b] = dst + 1;
+      const int slot = atomicAdd(&s_n, 1);
+      s_src[slot] = src;
+      s_dst[slot] = dst;
+      s_len[slot] = sz;
+    }
+    __syncthreads();
+    // payload copy: one wave per stream, lanes over bytes
+    for (int i = wave; i < s_n; i += blockDim.x >> 6) {
+      const uint8_t* sp = arena + s_src[i];
+      uint8_t* dp = blob + s_dst[i];
+      for (uint32_t j = lane; j < s_len[i]; j += 64) dp[j] = sp[j];
+    }
+    __syncthreads();
+    if (tid == 255) { s_vboff += s1[255]; s_doff += s2[255]; }
+    __syncthreads();
+  }
+}
+
+}  // namespace idx2g

@@ -1,0 +1,73 @@
+// Host-side objects of the idx2g library: context, resident dataset.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/idx2g.h"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace idx2g {
+
+struct LevelData {
+  int nb = 0;                  // existing bricks (dense, linear-id order)
+  std::vector<uint64_t> lin;
+  std::vector<int> xyz;
+  int klo = 0, nk = 0;         // bit-plane key range of the level
+  std::vector<ChunkRec> chunks;  // non-empty chunks, blob offsets filled
+  uint8_t* d_blob = nullptr;   // chunk bytes (+ padding)
+  uint64_t blob_bytes = 0;
+  uint64_t* d_dir = nullptr;   // [nsb][nk][span] stream byte offset + 1
+  uint8_t* d_exps = nullptr;   // per subband [dense][ncoded] exponent bytes
+  long long exp_off[8] = {0};
+  uint64_t exp_bytes = 0;
+  int* d_xyz = nullptr;
+};
+
+struct Context;
+
+}  // namespace idx2g
+
+struct idx2g_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  idx2g_stats last{};
+  int ncu = 256;
+  // timing events
+  hipEvent_t ev[8];
+};
+
+struct idx2g_dataset {
+  idx2g::Layout L;
+  int device = 0;
+  idx2g::LevelData lv[idx2g::kMaxLevels];
+  uint32_t* d_blocks = nullptr;  // packed block tables
+  int blk_off[idx2g::kMaxLevels][8];
+  double* d_minmax = nullptr;    // per level-0 brick min/max (encode only)
+  bool have_range = false;
+  uint64_t chunk_bytes = 0;
+  // decode scratch kept across queries
+  double* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+namespace idx2g {
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, int line);
+#define HIPCHK(x)                                                  \
+  do {                                                             \
+    hipError_t e_ = (x);                                           \
+    if (e_ != hipSuccess) return ::idx2g::hip_fail(e_, #x, __LINE__); \
+  } while (0)
+
+BrickConst make_brick_const(const Layout& L);
+PermTables make_perm_tables();
+int upload_block_tables(idx2g_dataset* ds);
+
+// zstd through the system libzstd.so.1
+int zstd_compress(const void* src, size_t n, std::vector<uint8_t>* out);
+int zstd_decompress(const void* src, size_t n, std::vector<uint8_t>* out);
+}  // namespace idx2g

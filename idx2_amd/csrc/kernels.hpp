@@ -1,0 +1,108 @@
+// Kernel argument blocks shared by the host drivers and the HIP kernels.
+#pragma once
+
+#include "device.hpp"
+
+namespace idx2g {
+
+constexpr int kThreads = 256;    // threads per brick workgroup (4 waves)
+constexpr int kMaxBlocks = 125;  // 4^d blocks per subband (17^3 subband of a 32^3 brick)
+constexpr int kMaxKeys = 128;    // bit-plane keys per brick-subband
+constexpr int kStageWords = 4096;  // LDS staging for one subband's streams (16 KB)
+constexpr short kSkip = -32768;
+
+// block table entry: org(5 bits x3) | dims(3 bits x3) << 15 | skip << 24 | coded index << 25
+__host__ __device__ inline uint32_t pack_block(const int org[3], const int dims[3], int skip, int ci) {
+  return uint32_t(org[0] | (org[1] << 5) | (org[2] << 10) | (dims[0] << 15) | (dims[1] << 18) |
+                  (dims[2] << 21) | (skip << 24)) | (uint32_t(ci) << 25);
+}
+
+struct SbEntry {  // one (brick, subband, key) stream in the arena
+  u64 off;
+  uint32_t size;
+  uint16_t first_block;
+  uint16_t key;
+};
+
+struct SbHdr {  // per (brick, subband)
+  int klo, nkeys;
+  u64 ebase;
+};
+
+struct EncArgs {
+  BrickConst k;
+  PermTables perm;
+  int level, coarsest;
+  // level 0 input
+  const void* input;
+  int dims[3], brick[3];
+  // bricks of this level, dense (linear-id) order
+  const int* xyz;       // [nb][3]
+  int nb, b0;           // process dense bricks [b0, b0+nb)
+  int nbricks_prev[3];  // bricks per axis of the level below (level > 0)
+  double* bricks;       // level > 0: accumulated bricks [dense][nvox] (transformed in place)
+  double* scratch;      // level 0: [gridDim.x][nvox]
+  // parents (non-coarsest)
+  double* parents;      // [dense parent][nvox]
+  const int* parent_of; // dense parent index per dense brick
+  int sbne[3];          // non-extended subband-0 dims
+  // block tables
+  const uint32_t* blocks;
+  int blk_off[8], blk_n[8], ncoded[8];
+  double wfwd[8];
+  int etol;
+  // outputs
+  uint8_t* exps;          // per subband [dense][ncoded] of 1 (f32) or 2 (f64) bytes
+  long long exp_off[8];
+  uint8_t* arena;
+  unsigned long long* counters;  // [0] arena bytes, [1] entries, [2] error flags
+  unsigned long long arena_cap, entry_cap;
+  SbEntry* entries;
+  SbHdr* hdrs;          // [dense][nsb]
+  double* minmax;       // [dense][2] (level 0)
+};
+
+// per (subband, key, chunk) assembled chunk
+struct ChunkRec {
+  u64 off;          // byte offset in the chunk blob
+  uint32_t size;    // bytes (0: no chunk)
+  uint32_t nbricks;
+  u64 first_brick;  // linear id of the first brick
+  u64 last_brick;
+  uint32_t first_fb;  // first block (Morton rank) of key in first_brick
+  int sb, key, chunk;
+};
+
+struct DecArgs {
+  BrickConst k;
+  PermTables perm;
+  int level, coarsest;
+  const int* xyz;        // [n][3] brick coordinates to decode (batch)
+  const u64* lin;        // [n] linear ids
+  const int* dense;      // [n] dense ids (exponent rows)
+  int nb;
+  double* bricks;        // [n][nvox] output bricks of this batch
+  const double* parents; // decoded bricks of level+1
+  const int* parent_slot;// [n] slot in parents
+  int sbne[3];
+  const uint32_t* blocks;
+  int blk_off[8], blk_n[8], ncoded[8];
+  double winv[8];
+  const uint8_t* blob;
+  const u64* dir;        // [nsb][nkeys][span] stream byte offset + 1, 0 = absent
+  int dir_klo, dir_nk;
+  u64 span;              // linear id span of the level
+  const uint8_t* exps;
+  long long exp_off[8];
+  int mask;
+  int spacing[8][3];
+  int etol;
+  // output grid (only when out != nullptr)
+  void* out;
+  int og_f[3], og_d[3], og_s[3];
+  int bstride;           // 1 << level
+  int brick[3], B3[3];   // brick dims and brick extent in samples at this level
+  unsigned long long* counters;  // [0] error flags
+};
+
+}  // namespace idx2g

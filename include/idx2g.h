@@ -1,0 +1,154 @@
+/* idx2g: MI355X (gfx950) implementation of the idx2 encode/decode path.
+ *
+ * C ABI: plain pointers and sizes, no C++ or torch types.  Every call returns
+ * an idx2 error code (numbering of the reference enum idx2_err_code,
+ * Source/Core/idx2Common.h:36-56 after the common codes of
+ * Source/Core/ErrorCodes.h:6-12); 0 is success.  No call throws.
+ *
+ * A context owns one HIP stream and the device pools on one GPU.  Contexts are
+ * not thread safe; use one per host thread.  Datasets own their device memory.
+ */
+#ifndef IDX2G_H
+#define IDX2G_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum idx2g_err {
+  IDX2G_OK = 0,
+  IDX2G_UNKNOWN = 1,
+  IDX2G_SIZE_ZERO = 2,
+  IDX2G_SIZE_TOO_SMALL = 3,
+  IDX2G_DIMENSION_MISMATCHED = 4,
+  IDX2G_SIZE_MISMATCHED = 5,
+  IDX2G_ALIGNMENT_ERROR = 6,
+  IDX2G_OUT_OF_MEMORY = 7,
+  IDX2G_TYPE_NOT_SUPPORTED = 8,
+  IDX2G_FILE_READ_FAILED = 11,
+  IDX2G_FILE_WRITE_FAILED = 12,
+  IDX2G_FILE_CREATE_FAILED = 13,
+  IDX2G_PARSE_FAILED = 17,
+  IDX2G_BRICK_SIZE_NOT_POW2 = 21,
+  IDX2G_BRICK_SIZE_TOO_BIG = 22,
+  IDX2G_TOO_MANY_LEVELS = 23,
+  IDX2G_TOO_MANY_FILES_PER_DIR = 27,
+  IDX2G_NOT_SUPPORTED = 28,
+  IDX2G_CANNOT_CREATE_DIRECTORY = 29,
+  IDX2G_SYNTAX_ERROR = 30,
+  IDX2G_TOO_MANY_BRICKS_PER_CHUNK = 31,
+  IDX2G_TOO_MANY_CHUNKS_PER_FILE = 32,
+  IDX2G_CHUNKS_PER_FILE_NOT_POW2 = 33,
+  IDX2G_BRICKS_PER_CHUNK_NOT_POW2 = 34,
+  IDX2G_CHUNK_NOT_FOUND = 35,
+  IDX2G_BRICK_NOT_FOUND = 36,
+  IDX2G_FILE_NOT_FOUND = 37,
+  IDX2G_DEVICE_ERROR = 100, /* HIP runtime failure; see idx2g_last_error_message */
+};
+
+typedef struct idx2g_ctx idx2g_ctx;
+typedef struct idx2g_dataset idx2g_dataset;
+
+/* Encode parameters: the fields of idx2::params used by encode
+ * (Source/Core/idx2Common.h:74-108, parsed by Applications/idx2App.cpp:114-168).
+ * Zero/NULL fields take the reference defaults (brick 32^3, 4096 bricks per
+ * chunk, 64 chunks per file, 64 files per directory, 1 bit plane per chunk,
+ * tolerance 0 = machine epsilon of the dtype, nlevels 0 = guessed). */
+typedef struct {
+  const char* name;        /* dataset name, <= 63 chars */
+  const char* field;       /* field name, <= 63 chars */
+  int32_t dims[3];         /* x, y, z; x fastest in memory */
+  int32_t dtype;           /* 0 = float32, 1 = float64 */
+  int32_t brick[3];
+  int32_t nlevels;
+  double tolerance;
+  int32_t bricks_per_chunk;
+  int32_t chunks_per_file;
+  int32_t files_per_dir;
+  int32_t bit_planes_per_chunk; /* only 1 is supported (as the reference's decoder) */
+} idx2g_encode_params;
+
+/* Decode query: the decode fields of idx2::params (idx2Common.h:91-97,
+ * Applications/idx2App.cpp:17-57).  first/last inclusive; first[0] < 0 means
+ * the whole volume. */
+typedef struct {
+  int32_t first[3];
+  int32_t last[3];
+  int32_t downsampling[3];
+  double tolerance;
+} idx2g_query;
+
+/* Per-call device timings (milliseconds, HIP events on the context stream). */
+typedef struct {
+  double total_ms;        /* first to last device operation of the call */
+  double transform_ms;    /* encode: transform+coding kernels; decode: decode kernels */
+  double assemble_ms;     /* encode: chunk assembly kernels; decode: output/copy kernels */
+  int64_t dominant_launches;
+  double dominant_ms;     /* summed duration of the dominant kernel's launches */
+  int64_t raw_bytes;      /* encode: input bytes; decode: output grid bytes */
+  int64_t chunk_bytes;    /* encode: bit-plane chunk bytes produced; decode: chunk bytes touched */
+  int64_t exp_bytes;      /* exponent bytes (uncompressed) produced / read */
+} idx2g_stats;
+
+/* ---- context ---- */
+int idx2g_create(idx2g_ctx** ctx, int hip_device);
+int idx2g_destroy(idx2g_ctx* ctx);
+const char* idx2g_last_error_message(void);
+
+/* ---- encode ----
+ * Replaces idx2::Encode(idx2_file*, const params&, brick_copier&)
+ * (Source/Core/idx2Encode.h:143-144, Source/idx2.h:29-30).  The input grid is
+ * read from host memory (input_is_device = 0) or device memory (1).  The
+ * result stays resident on the device as a dataset (chunks assembled in HBM);
+ * idx2g_dataset_write writes the .idx2 file and the L*.bin files. */
+int idx2g_encode(idx2g_ctx* ctx, const idx2g_encode_params* p, const void* input,
+                 int input_is_device, idx2g_dataset** out);
+
+/* Write out_dir/name/field.idx2 and out_dir/name/field/L..bin with the file
+ * layout of FlushChunks/FlushChunkExponents/WriteMetaFile/PrintStats
+ * (Source/Core/idx2Write.cpp:30-413, idx2Common.cpp:149-179). */
+int idx2g_dataset_write(idx2g_dataset* ds, const char* out_dir);
+
+/* ---- open ----
+ * Replaces idx2::Init(idx2_file*, params&) (Source/idx2.h:20-21,
+ * Source/idx2.cpp:23-32): reads the .idx2 metadata and every L*.bin file of
+ * the dataset and makes chunks, chunk index and exponents resident in HBM. */
+int idx2g_open(idx2g_ctx* ctx, const char* idx2_path, const char* in_dir, idx2g_dataset** out);
+int idx2g_dataset_free(idx2g_dataset* ds);
+
+/* Dataset properties: dims, dtype, levels, brick; bytes of chunk data resident. */
+int idx2g_dataset_info(const idx2g_dataset* ds, int32_t dims[3], int32_t* dtype, int32_t* nlevels,
+                       int32_t brick[3], double* tolerance, int64_t* chunk_bytes);
+
+/* ---- decode ----
+ * Output grid of a query: replaces idx2::GetOutputGrid (Source/idx2.h:36-37,
+ * Source/Core/idx2Common.cpp:770-785). */
+int idx2g_output_dims(const idx2g_dataset* ds, const idx2g_query* q, int64_t dims[3], int64_t* bytes);
+
+/* Replaces idx2::Decode(idx2_file*, params&, buffer*) (Source/idx2.h:43-44,
+ * Source/Core/idx2Decode.cpp:326-492) in RegularGridMem mode: decodes into a
+ * caller-allocated buffer of idx2g_output_dims bytes (host or device).  The
+ * buffer is zeroed first, so samples outside every brick read 0. */
+int idx2g_decode(idx2g_ctx* ctx, idx2g_dataset* ds, const idx2g_query* q, void* out,
+                 int out_is_device);
+
+/* Device timings of the last encode/decode call on ctx. */
+int idx2g_last_stats(const idx2g_ctx* ctx, idx2g_stats* s);
+
+/* Host copy of a dataset's bit-plane chunk bytes in file order: for each file
+ * its path suffix, and the concatenated chunk payloads (for parity tests).
+ * Call with buf == NULL to get the size. */
+int idx2g_dataset_chunk_bytes(idx2g_dataset* ds, uint8_t* buf, int64_t* nbytes);
+
+/* ---- synthetic input ----
+ * Fills a device buffer with the deterministic Gaussian+sine+noise field of
+ * idx2_amd/synth.py (same bytes).  params: 192 doubles (synth.field_params). */
+int idx2g_synth_field(idx2g_ctx* ctx, void* dev_out, const int32_t dims[3], int32_t dtype,
+                      const double* params, uint64_t noise_seed, int32_t z0, int32_t z1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,103 @@
+"""Write golden fixtures from the reference idx2App (built from the reference
+sources into oracle/_ref by oracle/ref.mk).
+
+For each case: the input is regenerated from (dims, dtype, seed) by
+idx2_amd.synth (its sha256 is recorded to catch generator drift); the
+reference's encoded dataset (.idx2 + L*.bin files) is copied verbatim; and for
+a set of decode queries the sha256 and a few samples of the reference's
+decoded grid are recorded.
+
+Run in the container that has the reference sources:
+    make -f oracle/ref.mk && python tests/golden/make_fixtures.py
+"""
+import hashlib
+import json
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from idx2_amd import synth  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+CASES = {
+    "c3d_f32_64": dict(dims=(64, 64, 64), dtype="float32", nlevels=1, tolerance=1e-3,
+                       brick=(32, 32, 32), bpc=4096, cpf=64, fpd=64, seed=7),
+    "c3d_f64_partial": dict(dims=(48, 40, 36), dtype="float64", nlevels=1, tolerance=1e-4,
+                            brick=(32, 32, 32), bpc=2, cpf=2, fpd=2, seed=3),
+    "c3d_f32_2lvl": dict(dims=(72, 68, 66), dtype="float32", nlevels=2, tolerance=1e-2,
+                         brick=(32, 32, 32), bpc=8, cpf=2, fpd=2, seed=5),
+    "c3d_f64_b16": dict(dims=(40, 36, 34), dtype="float64", nlevels=2, tolerance=1e-6,
+                        brick=(16, 16, 16), bpc=8, cpf=4, fpd=4, seed=9),
+    "c2d_f64": dict(dims=(128, 96, 1), dtype="float64", nlevels=2, tolerance=1e-4,
+                    brick=(32, 32, 1), bpc=4096, cpf=64, fpd=64, seed=1),
+    "c2d_f32_3lvl": dict(dims=(300, 260, 1), dtype="float32", nlevels=3, tolerance=1e-4,
+                         brick=(32, 32, 1), bpc=4, cpf=2, fpd=4, seed=2),
+}
+
+
+def queries(dims):
+    nx, ny, nz = dims
+    is2d = nz == 1
+    q = [dict(),
+         dict(tolerance=1e-2),
+         dict(downsampling=(1, 1, 0 if is2d else 1)),
+         dict(downsampling=(1, 0, 0 if is2d else 2), tolerance=1e-3),
+         dict(first=(3, 5, 0 if is2d else 2), last=(min(40, nx - 1), min(30, ny - 1), 0 if is2d else min(25, nz - 1))),
+         dict(first=(7, 3, 0 if is2d else 1), last=(nx - 2, ny - 5, 0 if is2d else nz - 3), downsampling=(1, 1, 0 if is2d else 1)),
+         dict(downsampling=(2, 2, 0 if is2d else 2)),
+         ]
+    return q
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).view(np.uint8).tobytes()).hexdigest()
+
+
+def make_case(name, spec):
+    out = os.path.join(HERE, name)
+    shutil.rmtree(out, ignore_errors=True)
+    os.makedirs(out)
+    data = synth.make_field(spec["dims"], spec["dtype"], seed=spec["seed"])
+    with tempfile.TemporaryDirectory() as wd:
+        rc, rdir = O.ref_encode(wd, "G", "f", data, brick=spec["brick"], nlevels=spec["nlevels"],
+                                tolerance=spec["tolerance"], bricks_per_chunk=spec["bpc"],
+                                chunks_per_file=spec["cpf"], files_per_dir=spec["fpd"])
+        assert rc == 0, rc
+        shutil.copytree(os.path.join(rdir, "G"), os.path.join(out, "G"))
+        idx = os.path.join(rdir, "G", "f.idx2")
+        qs = []
+        for q in queries(spec["dims"]):
+            rc, path = O.ref_decode(wd, idx, **q)
+            assert rc == 0, (name, q)
+            dec = np.fromfile(path, dtype=data.dtype)
+            rec = {k: list(v) if isinstance(v, tuple) else v for k, v in q.items()}
+            rec["sha256"] = sha(dec)
+            rec["size"] = int(dec.size)
+            idxs = np.linspace(0, dec.size - 1, 8).astype(np.int64)
+            rec["samples"] = [[int(i), float(dec[i])] for i in idxs]
+            if not q:
+                rec["max_abs_err"] = float(np.max(np.abs(dec.astype(np.float64) - data.reshape(-1).astype(np.float64))))
+            qs.append(rec)
+    meta = dict(spec)
+    meta["input_sha256"] = sha(data)
+    meta["queries"] = qs
+    with open(os.path.join(out, "case.json"), "w") as f:
+        json.dump(meta, f, indent=1)
+
+
+def main():
+    assert O.ref_available(), "build the reference first: make -f oracle/ref.mk"
+    for name, spec in CASES.items():
+        make_case(name, spec)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()
