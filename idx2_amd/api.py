@@ -47,7 +47,7 @@ class Query(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("transform_ms", ctypes.c_double),
                 ("assemble_ms", ctypes.c_double), ("dominant_launches", ctypes.c_int64),
-                ("dominant_ms", ctypes.c_double), ("raw_bytes", ctypes.c_int64),
+                ("dominant_ms", ctypes.c_double), ("dominant_bytes", ctypes.c_int64), ("raw_bytes", ctypes.c_int64),
                 ("chunk_bytes", ctypes.c_int64), ("exp_bytes", ctypes.c_int64)]
 
 

@@ -96,7 +96,7 @@ struct EncodeBuffers {
 };
 
 static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_input, EncodeBuffers& B,
-                             unsigned long long* flags) {
+                             unsigned long long* flags, unsigned long long* l0_bytes) {
   const Layout& L = ds->L;
   const BrickConst kc = make_brick_const(L);
   const PermTables pt = make_perm_tables();
@@ -178,8 +178,13 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
     a.hdrs = B.hdrs[l];
     a.minmax = ds->d_minmax;
     const int grid = l == 0 ? grid0 : std::min(lv.nb, ctx->ncu * 4);
+    if (l == 0) HIPCHK(hipEventRecord(ctx->ev[3], st));
     hipLaunchKernelGGL(k_encode_level, dim3(grid), dim3(kThreads), lds, st, a);
     HIPCHK(hipGetLastError());
+    if (l == 0) {
+      HIPCHK(hipEventRecord(ctx->ev[4], st));
+      HIPCHK(hipMemcpyAsync(l0_bytes, B.counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    }
     if (bricks_cur) {
       HIPCHK(hipStreamSynchronize(st));
       (void)hipFree(bricks_cur);
@@ -323,8 +328,9 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
     TRY(dalloc(&B.entries, B.entry_cap));
     TRY(dalloc(&B.counters, 4));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
-    unsigned long long flags[3];
-    rc = run_encode_levels(ctx, ds, d_in, B, flags);
+    unsigned long long flags[3], l0 = 0;
+    rc = run_encode_levels(ctx, ds, d_in, B, flags, &l0);
+    stats.dominant_bytes = (int64_t)(raw + l0);
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     if (rc) break;
     if (flags[2] & 1) {
@@ -358,6 +364,10 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
   (void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
   (void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
   (void)hipEventElapsedTime(&t02, ctx->ev[0], ctx->ev[2]);
+  float td = 0;
+  (void)hipEventElapsedTime(&td, ctx->ev[3], ctx->ev[4]);
+  stats.dominant_ms = td;
+  stats.dominant_launches = 1;
   stats.transform_ms = t01;
   stats.assemble_ms = t12;
   stats.total_ms = t02;

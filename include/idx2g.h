@@ -86,7 +86,9 @@ typedef struct {
   double transform_ms;    /* encode: transform+coding kernels; decode: decode kernels */
   double assemble_ms;     /* encode: chunk assembly kernels; decode: output/copy kernels */
   int64_t dominant_launches;
-  double dominant_ms;     /* summed duration of the dominant kernel's launches */
+  double dominant_ms;     /* summed duration of the dominant kernel's launches (level-0 brick kernel) */
+  int64_t dominant_bytes; /* algorithmic bytes of those launches: encode raw input + streams written;
+                             decode chunk streams read + output grid written */
   int64_t raw_bytes;      /* encode: input bytes; decode: output grid bytes */
   int64_t chunk_bytes;    /* encode: bit-plane chunk bytes produced; decode: chunk bytes touched */
   int64_t exp_bytes;      /* exponent bytes (uncompressed) produced / read */

@@ -860,6 +860,12 @@ static void zfp_ilift(i64* p, int s)
   w += x; x <<= 1; x -= w;
   p[0] = x; p[s] = y; p[2 * s] = z; p[3 * s] = w;
 }
+void idx2o_zfp_lift(i64* p, int fwd)
+{
+  if (fwd) zfp_flift(p, 1);
+  else zfp_ilift(p, 1);
+}
+
 /* ForwardZfp(P, D), Zfp.h:261-295 */
 static void zfp_forward(i64* p, int D)
 {

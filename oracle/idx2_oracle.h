@@ -68,6 +68,8 @@ void idx2o_block_inverse(const uint64_t* in, int ndims, int emax, double* out);
 int idx2o_encode_plane(const uint64_t* block, int nvals, int bp, int* n, uint8_t* out, int64_t bitpos);
 int idx2o_decode_plane(uint64_t* block, int nvals, int bp, int* n, const uint8_t* in, int64_t bitpos);
 void idx2o_cdf53_norms(double scal[16], double wave[16]);
+/* 4-point zfp lifting (FLift/ILift, Zfp.h:144-204) on p[0..3] */
+void idx2o_zfp_lift(int64_t* p, int fwd);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,7 @@ def lib(variant=None):
         L.idx2o_decode_plane.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_int), ctypes.c_void_p, ctypes.c_int64]
         L.idx2o_cdf53_norms.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.idx2o_zfp_lift.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _libs[variant] = L
     return _libs[variant]
 

@@ -1,0 +1,179 @@
+"""HIP path parity, called through the C ABI on the GPU.
+
+* encode: files byte-identical with the oracle (same system zstd) and
+  section-identical with the reference's golden files (bit-plane chunks,
+  addresses, decompressed exponent streams);
+* decode: reference-written golden files decode bit-identically to the
+  reference's own decode (sha256 of every recorded query);
+* round trips and ROI/crop consistency at larger sizes.
+"""
+import glob
+import hashlib
+import os
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, golden_cases
+from idx2_amd import synth
+import idx2file
+
+pytestmark = pytest.mark.gpu
+CASES = golden_cases()
+IDS = [c[0] for c in CASES]
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).view(np.uint8).tobytes()).hexdigest()
+
+
+def _files(root):
+    return sorted(p[len(root):] for p in glob.glob(os.path.join(root, "**", "*.bin"), recursive=True))
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from idx2_amd import api
+    c = api.Context(0)
+    yield c
+    c.close()
+
+
+def _kw(spec):
+    return dict(brick=spec["brick"], nlevels=spec["nlevels"], tolerance=spec["tolerance"],
+                bricks_per_chunk=spec["bpc"], chunks_per_file=spec["cpf"], files_per_dir=spec["fpd"])
+
+
+@pytest.mark.parametrize("name,spec", CASES, ids=IDS)
+def test_encode_matches_reference_and_oracle(ctx, oracle_lib, name, spec):
+    from idx2_amd import api
+    print(f"case {name}", flush=True)
+    data = synth.make_field(spec["dims"], spec["dtype"], seed=spec["seed"])
+    gold = os.path.join(GOLDEN, name, "G", "f")
+    with tempfile.TemporaryDirectory() as wd:
+        ds = api.Dataset.encode(ctx, data, name="G", field="f", **_kw(spec))
+        ds.write(os.path.join(wd, "gpu"))
+        ds.free()
+        mine = os.path.join(wd, "gpu", "G", "f")
+        assert _files(mine) == _files(gold)
+        for rel in _files(gold):
+            assert idx2file.same_content(open(gold + rel, "rb").read(), open(mine + rel, "rb").read()), rel
+        oracle_lib.use_variant("system")
+        assert oracle_lib.encode(os.path.join(wd, "ora"), "G", "f", data, **_kw(spec)) == 0
+        ora = os.path.join(wd, "ora", "G", "f")
+        for rel in _files(ora):
+            assert open(ora + rel, "rb").read() == open(mine + rel, "rb").read(), rel
+        assert open(os.path.join(wd, "ora", "G", "f.idx2")).read() == open(os.path.join(wd, "gpu", "G", "f.idx2")).read()
+
+
+@pytest.mark.parametrize("name,spec", CASES, ids=IDS)
+def test_decode_reference_files(ctx, name, spec):
+    from idx2_amd import api
+    print(f"case {name}", flush=True)
+    ds = api.Dataset.open(ctx, os.path.join(GOLDEN, name, "G", "f.idx2"))
+    for q in spec["queries"]:
+        kw = {k: tuple(v) if isinstance(v, list) else v for k, v in q.items()
+              if k in ("first", "last", "downsampling", "tolerance")}
+        out = ds.decode(**kw)
+        assert out.size == q["size"]
+        assert _sha(out) == q["sha256"], q
+    ds.free()
+
+
+def _roundtrip_case(ctx, oracle_lib, dims, dtype, nlevels, tol, brick, queries, seed=11):
+    from idx2_amd import api
+    data = synth.make_field(dims, dtype, seed=seed)
+    with tempfile.TemporaryDirectory() as wd:
+        ds = api.Dataset.encode(ctx, data, name="R", field="f", brick=brick, nlevels=nlevels, tolerance=tol)
+        ds.write(wd)
+        oracle_lib.use_variant("system")
+        assert oracle_lib.encode(os.path.join(wd, "ora"), "R", "f", data, brick=brick, nlevels=nlevels,
+                                 tolerance=tol) == 0
+        for rel in _files(os.path.join(wd, "ora", "R", "f")):
+            assert open(os.path.join(wd, "ora", "R", "f") + rel, "rb").read() == \
+                open(os.path.join(wd, "R", "f") + rel, "rb").read(), rel
+        for q in queries:
+            g = ds.decode(**q)
+            rc, o = oracle_lib.decode(os.path.join(wd, "R", "f.idx2"), wd, **q)
+            assert rc == 0 and g.shape == o.shape
+            assert np.array_equal(g.view(np.uint8), o.view(np.uint8)), q
+        ds.free()
+    return data
+
+
+def test_larger_3d_f32_three_levels(ctx, oracle_lib):
+    print("case 256x256x160 f32 3 levels", flush=True)
+    _roundtrip_case(ctx, oracle_lib, (256, 256, 160), "float32", 3, 1e-4, (32, 32, 32),
+                    [dict(), dict(downsampling=(1, 1, 1)), dict(tolerance=1e-2),
+                     dict(first=(17, 40, 3), last=(200, 131, 150), downsampling=(0, 1, 0))])
+
+
+def test_2d_f64_slice(ctx, oracle_lib):
+    print("case 1024x768x1 f64 3 levels", flush=True)
+    _roundtrip_case(ctx, oracle_lib, (1024, 768, 1), "float64", 3, 1e-4, (32, 32, 1),
+                    [dict(), dict(downsampling=(2, 2, 0)), dict(first=(100, 37, 0), last=(900, 700, 0))])
+
+
+def test_roi_is_crop_of_full_decode(ctx):
+    from idx2_amd import api
+    print("case roi crop", flush=True)
+    data = synth.make_field((160, 128, 96), "float32", seed=2)
+    ds = api.Dataset.encode(ctx, data, brick=(32, 32, 32), nlevels=2, tolerance=1e-4)
+    full = ds.decode()
+    f, l_ = (13, 7, 20), (150, 99, 60)
+    roi = ds.decode(first=f, last=l_)
+    assert np.array_equal(roi, full[f[2]:l_[2] + 1, f[1]:l_[1] + 1, f[0]:l_[0] + 1])
+    ds.free()
+
+
+def test_reference_decodes_gpu_files(ctx, oracle_lib):
+    """The reference idx2App (oracle/_ref) reads files the GPU wrote."""
+    from idx2_amd import api
+    if not oracle_lib.ref_available():
+        pytest.skip("reference build not present")
+    print("case reference reads gpu files", flush=True)
+    data = synth.make_field((96, 80, 70), "float64", seed=4)
+    with tempfile.TemporaryDirectory() as wd:
+        ds = api.Dataset.encode(ctx, data, name="X", field="f", brick=(32, 32, 32), nlevels=1, tolerance=1e-5)
+        ds.write(wd)
+        g = ds.decode(downsampling=(1, 0, 1), tolerance=1e-3)
+        rc, path = oracle_lib.ref_decode(wd, os.path.join(wd, "X", "f.idx2"), downsampling=(1, 0, 1),
+                                         tolerance=1e-3)
+        assert rc == 0
+        assert np.array_equal(np.fromfile(path, dtype=np.float64), g.reshape(-1))
+        ds.free()
+
+
+def test_invalid_parameters_return_reference_codes(ctx):
+    from idx2_amd import api
+    data = np.zeros((64, 64, 64), np.float32)
+    cases = [(dict(brick=(24, 32, 32)), 21), (dict(brick=(128, 32, 32)), 22), (dict(nlevels=3), 23),
+             (dict(bricks_per_chunk=3000), 34), (dict(chunks_per_file=3), 33), (dict(bricks_per_chunk=65536), 31)]
+    for kw, code in cases:
+        with pytest.raises(api.Idx2Error) as e:
+            api.Dataset.encode(ctx, data, **kw)
+        assert e.value.code == code, kw
+
+
+def test_missing_file_is_file_not_found(ctx):
+    from idx2_amd import api
+    with tempfile.TemporaryDirectory() as wd:
+        shutil.copytree(os.path.join(GOLDEN, "c3d_f32_64", "G"), os.path.join(wd, "G"))
+        os.remove(os.path.join(wd, "G", "f", "L00.bin"))
+        with pytest.raises(api.Idx2Error) as e:
+            api.Dataset.open(ctx, os.path.join(wd, "G", "f.idx2"))
+        assert e.value.code == 37
+
+
+def test_device_synth_matches_numpy(ctx):
+    import torch
+    from idx2_amd import api  # noqa: F401
+    for dims, dt in (((48, 40, 36), "float64"), ((70, 33, 21), "float32")):
+        ref = synth.make_field(dims, dt, seed=9)
+        buf = torch.empty(ref.size, dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+        ctx.synth_field(buf.data_ptr(), dims, dt, 9)
+        torch.cuda.synchronize()
+        assert np.array_equal(buf.cpu().numpy().view(np.uint8), ref.reshape(-1).view(np.uint8))
