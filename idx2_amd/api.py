@@ -59,9 +59,22 @@ EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_e
 _lib = None
 
 
+def _share_hip_runtime_with_torch():
+    """libidx2g.so needs libamdhip64.so.7.  When torch (which bundles its own
+    HIP runtime under that soname) is loaded first, the library binds to that
+    same runtime, so device pointers and streams are shared and only one HIP
+    runtime lives in the process.  Loading libidx2g.so first would bring in
+    a second runtime (/opt/rocm), after which torch cannot initialise HIP."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
+        _share_hip_runtime_with_torch()
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with idx2_amd/build.py")
         L = ctypes.CDLL(LIB_PATH)
