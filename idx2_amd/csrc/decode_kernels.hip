@@ -76,7 +76,8 @@ __device__ inline void dunpack(uint32_t w, int org[3], int dims[3], int* skip, i
   *ci = int(w >> 25);
 }
 
-__global__ void __launch_bounds__(kThreads) k_decode_level(DecArgs a) {
+__global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __restrict__ ap) {
+  const DecArgs& a = *ap;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ int s_klo, s_khi;
   const BrickConst& k = a.k;

@@ -8,12 +8,15 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "brick32.hpp"
 
 namespace idx2g {
 
 struct EncShared {
   int klo, khi, err;
-  double wmn[kThreads / 64], wmx[kThreads / 64];
+  double wmn[16], wmx[16];
+  unsigned long long base, ebase;
+  uint32_t total;
 };
 
 // LDS layout for one subband (dynamic shared memory), npl planes per block
@@ -87,245 +90,323 @@ __device__ inline void or_bits(uint32_t* words, u64 pos, const u64 w[3], int len
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_encode_level(EncArgs a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  __shared__ EncShared sh;
+// Per-subband zfp block stage and bit-plane coding of one transformed brick F
+// (EncodeSubbandBlocks, Source/Core/idx2Encode.cpp:141-256): stage A one wave
+// per block (quantise, lift, shuffle, one ballot per plane), B emitted
+// lengths, C per-key bit offsets in Morton order, D arena reservation,
+// E stream bits (LDS staging).
+__device__ void code_subbands(const EncArgs& a, const double* F, int dense, unsigned char* lds_raw,
+                              EncShared& sh) {
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
   EncLds S = carve(lds_raw, npl);
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int prec3[4] = {63, 62, 61, 60};
-
-  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
-    const int dense = a.b0 + bi;
-    const int b3[3] = {a.xyz[3 * dense], a.xyz[3 * dense + 1], a.xyz[3 * dense + 2]};
-    double* F;
-    int ext[3];
-    if (a.level == 0) {
-      // gather the brick into a zeroed extended volume (brick_copier::Copy,
-      // Source/Core/idx2Encode.cpp:377-386, Volume.h:1123-1149)
-      F = a.scratch + (size_t)blockIdx.x * k.nvox;
-      int from[3];
-      for (int d = 0; d < 3; ++d) {
-        from[d] = b3[d] * a.brick[d];
-        ext[d] = max(0, min(a.brick[d], a.dims[d] - from[d]));
+  for (int sb = 0; sb < k.nsb; ++sb) {
+    const DGrid& g = k.sb[sb];
+    const int nblk = a.blk_n[sb];
+    const uint32_t* blocks = a.blocks + a.blk_off[sb];
+    if (tid == 0) { sh.klo = 0x7fffffff; sh.khi = -0x7fffffff; sh.err = 0; }
+    __syncthreads();
+    // ---- stage A: one wave per block, one coefficient per lane ----
+    for (int b = wave; b < nblk; b += nwaves) {
+      int org[3], bd[3], skip, ci;
+      unpack_block(blocks[b], org, bd, &skip, &ci);
+      if (skip) {
+        if (lane == 0) { S.emax[b] = kSkip; S.np[b] = 0; }
+        continue;
       }
-      double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
-      for (int i = tid; i < k.nvox; i += blockDim.x) {
-        int x = i % k.N[0], y = (i / k.N[0]) % k.N[1], z = i / (k.N[0] * k.N[1]);
-        double v = 0;
-        if (x < ext[0] && y < ext[1] && z < ext[2]) {
-          size_t src = ((size_t)(from[2] + z) * a.dims[1] + (from[1] + y)) * (size_t)a.dims[0] + (from[0] + x);
-          v = k.dtype ? ((const double*)a.input)[src] : (double)((const float*)a.input)[src];
-          mn = v < mn ? v : mn;
-          mx = v > mx ? v : mx;
+      const int nd = (bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1);
+      const int nv = 1 << (2 * nd);
+      const bool act = lane < nv;
+      // gather order: X fastest over the block dims (idx2_BeginFor3)
+      int x = lane % bd[0], y = (lane / bd[0]) % bd[1], z = lane / (bd[0] * bd[1]);
+      double v = 0;
+      if (act)
+        v = F[vox(k, g.f[0] + g.s[0] * (org[0] + x), g.f[1] + g.s[1] * (org[1] + y),
+                  g.f[2] + g.s[2] * (org[2] + z))];
+      // QuantizeF32/F64 (Source/Core/Function.h:58-89)
+      const double maxabs = wave_max(act ? fabs(v) : 0.0);
+      const int emax = k.dtype ? exponent_f64(maxabs) : exponent_f32((float)maxabs);
+      const int prec = prec3[nd];
+      const double scale = pow2d(prec - 1 - emax);
+      i64 q = act ? f64_to_i64(scale * v) : 0;
+      q = zfp_forward_lanes(q, lane, nd);
+      // ForwardShuffle + negabinary (Zfp.h:534-559)
+      int src = nd == 3 ? a.perm.p3[lane] : nd == 2 ? a.perm.p2[lane & 15] : lane;
+      i64 qq = shfl_i64(q, src);
+      const u64 u = ((u64)qq + IDX2G_NBMASK) ^ IDX2G_NBMASK;
+      int stop = max(64 - k.endbp, 57 + a.etol - emax);
+      int np = stop > 63 ? 0 : 64 - stop;
+      for (int p = 0; p < np; ++p) {
+        u64 X = __ballot(act && ((u >> (63 - p)) & 1ull));
+        if (lane == 0) S.pw[b * npl + p] = X;
+      }
+      if (lane == 0) {
+        S.emax[b] = (short)emax;
+        S.np[b] = (unsigned char)np;
+        if (np > 0) {
+          atomicMax(&sh.khi, 63 + emax + kBpBiasDev);
+          atomicMin(&sh.klo, stop + emax + kBpBiasDev);
         }
-        F[i] = v;
-      }
-      mn = -wave_max(-mn);
-      mx = wave_max(mx);
-      if (lane == 0) { sh.wmn[wave] = mn; sh.wmx[wave] = mx; }
-      __syncthreads();
-      if (tid == 0) {
-        for (int w = 1; w < nwaves; ++w) {
-          mn = sh.wmn[w] < mn ? sh.wmn[w] : mn;
-          mx = sh.wmx[w] > mx ? sh.wmx[w] : mx;
-        }
-        a.minmax[2 * dense] = mn;
-        a.minmax[2 * dense + 1] = mx;
-      }
-    } else {
-      F = a.bricks + (size_t)dense * k.nvox;
-      for (int d = 0; d < 3; ++d) {
-        if (a.brick[d] > 1) {
-          int nch = min(2, a.nbricks_prev[d] - 2 * b3[d]);
-          ext[d] = nch * a.sbne[d];
+        // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
+        const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
+        if (k.dtype) {
+          uint16_t e = (uint16_t)(emax + 1023);
+          a.exps[row] = (uint8_t)(e & 0xff);
+          a.exps[row + 1] = (uint8_t)(e >> 8);
         } else {
-          ext[d] = 1;
+          a.exps[row] = (uint8_t)(emax + 127);
         }
       }
     }
     __syncthreads();
+    // ---- stage B: emitted length of every (block, plane), thread per block ----
+    for (int b = tid; b < nblk; b += blockDim.x) {
+      if (S.emax[b] == kSkip) continue;
+      int org[3], bd[3], skip, ci;
+      unpack_block(blocks[b], org, bd, &skip, &ci);
+      const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
+      int n = 0;
+      for (int p = 0; p < S.np[b]; ++p) S.plen[b * npl + p] = (uint16_t)plane_len(S.pw[b * npl + p], n, nv);
+    }
+    __syncthreads();
+    const int klo = sh.klo, khi = sh.khi;
+    const int nk = khi >= klo ? khi - klo + 1 : 0;
+    if (nk > kMaxKeys) {
+      if (tid == 0) { flag_error(a.counters, 1); a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0}; }
+      __syncthreads();
+      continue;
+    }
+    // ---- stage C: per key, bit offsets of each block's plane (Morton order) ----
+    for (int kk = tid; kk < nk; kk += blockDim.x) {
+      const int key = klo + kk;
+      uint32_t bits = 0;
+      int first = -1;
+      for (int b = 0; b < nblk; ++b) {
+        const int e = S.emax[b];
+        if (e == kSkip) continue;
+        const int p = 63 - (key - kBpBiasDev - e);
+        if (p >= 0 && p < S.np[b]) {
+          S.poff[b * npl + p] = bits;
+          bits += S.plen[b * npl + p];
+          if (first < 0) first = b;
+        }
+      }
+      S.kbits[kk] = bits;
+      S.kfirst[kk] = (short)first;
+    }
+    __syncthreads();
+    // ---- stage D: byte layout of the key streams, arena + entry reservation ----
+    if (tid == 0) {
+      uint32_t tot = 0;
+      for (int kk = 0; kk < nk; ++kk) {
+        S.kboff[kk] = tot;
+        tot += (S.kbits[kk] + 7) >> 3;
+      }
+      sh.total = tot;
+      const unsigned long long alloc = (tot + 3ull) & ~3ull;
+      sh.base = alloc ? atomicAdd(&a.counters[0], alloc) : 0;
+      sh.ebase = nk ? atomicAdd(&a.counters[1], (unsigned long long)nk) : 0;
+      const bool over = sh.base + alloc > a.arena_cap || sh.ebase + nk > a.entry_cap;
+      if (over) flag_error(a.counters, 2);
+      SbHdr h;
+      h.klo = klo;
+      h.nkeys = over ? 0 : nk;
+      h.ebase = sh.ebase;
+      a.hdrs[(size_t)dense * k.nsb + sb] = h;
+    }
+    __syncthreads();
+    const unsigned long long base = sh.base;
+    const uint32_t total = sh.total;
+    if (base + ((total + 3ull) & ~3ull) > a.arena_cap || sh.ebase + nk > a.entry_cap) continue;
+    for (int kk = tid; kk < nk; kk += blockDim.x) {
+      SbEntry e;
+      e.off = base + S.kboff[kk];
+      e.size = (S.kbits[kk] + 7) >> 3;
+      e.first_block = (uint16_t)(S.kfirst[kk] < 0 ? 0xffff : S.kfirst[kk]);
+      e.key = (uint16_t)(klo + kk);
+      a.entries[sh.ebase + kk] = e;
+    }
+    // ---- stage E: write the streams (LDS staging, else global) ----
+    const uint32_t nwords = (total + 3) >> 2;
+    const bool staged = nwords <= (uint32_t)kStageWords;
+    uint32_t* gwords = (uint32_t*)(a.arena + base);
+    if (staged) {
+      for (uint32_t i = tid; i < nwords; i += blockDim.x) S.stage[i] = 0;
+    } else {
+      for (uint32_t i = tid; i < nwords; i += blockDim.x) gwords[i] = 0;
+      __threadfence();
+    }
+    __syncthreads();
+    for (int b = tid; b < nblk; b += blockDim.x) {
+      const int e = S.emax[b];
+      if (e == kSkip) continue;
+      int org[3], bd[3], skip, ci;
+      unpack_block(blocks[b], org, bd, &skip, &ci);
+      const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
+      int n = 0;
+      for (int p = 0; p < S.np[b]; ++p) {
+        const u64 X = S.pw[b * npl + p];
+        u64 w[3];
+        const int len = plane_emit(X, n, nv, w);
+        plane_len(X, n, nv);  // advance n
+        const int kk = (63 - p) + e + kBpBiasDev - klo;
+        const u64 pos = (u64)S.kboff[kk] * 8 + S.poff[b * npl + p];
+        if (staged) or_bits<true>(S.stage, pos, w, len);
+        else or_bits<false>(gwords, pos, w, len);
+      }
+    }
+    __syncthreads();
+    if (staged)
+      for (uint32_t i = tid; i < nwords; i += blockDim.x) gwords[i] = S.stage[i];
+    __syncthreads();
+  }
+}
+
+// Gather a brick from a level grid (the input at level 0, the dense f64
+// grid of subband-0 interiors above) into a zeroed extended volume
+// (brick_copier::Copy, Source/Core/idx2Encode.cpp:377-386, Volume.h:1123-1149).
+__device__ inline void brick_origin(const EncArgs& a, int dense, int b3[3], int from[3], int ext[3]) {
+  for (int d = 0; d < 3; ++d) {
+    b3[d] = a.xyz[3 * dense + d];
+    from[d] = b3[d] * a.brick[d];
+    ext[d] = max(0, min(a.brick[d], a.gdims[d] - from[d]));
+  }
+}
+
+__device__ inline double load_sample(const EncArgs& a, size_t i) {
+  return a.in_dtype ? ((const double*)a.input)[i] : (double)((const float*)a.input)[i];
+}
+
+__device__ inline void store_minmax(const EncArgs& a, EncShared& sh, int dense, double mn, double mx) {
+  const int lane = lane_id(), wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  mn = -wave_max(-mn);
+  mx = wave_max(mx);
+  if (lane == 0) { sh.wmn[wave] = mn; sh.wmx[wave] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nwaves; ++w) {
+      mn = sh.wmn[w] < mn ? sh.wmn[w] : mn;
+      mx = sh.wmx[w] > mx ? sh.wmx[w] : mx;
+    }
+    a.minmax[2 * dense] = mn;
+    a.minmax[2 * dense + 1] = mx;
+  }
+}
+
+// Any brick (partial bricks at the volume's far faces, any brick size up to
+// 32): the extended brick lives in a global scratch volume per workgroup.
+__global__ void __launch_bounds__(kThreads) k_encode_level(const EncArgs* __restrict__ ap) {
+  const EncArgs& a = *ap;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ EncShared sh;
+  const BrickConst& k = a.k;
+  const int tid = threadIdx.x;
+  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+    const int dense = a.list ? a.list[bi] : a.b0 + bi;
+    int b3[3], from[3], ext[3];
+    brick_origin(a, dense, b3, from, ext);
+    double* F = a.scratch + (size_t)blockIdx.x * k.nvox;
+    double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+    for (int i = tid; i < k.nvox; i += blockDim.x) {
+      int x = i % k.N[0], y = (i / k.N[0]) % k.N[1], z = i / (k.N[0] * k.N[1]);
+      double v = 0;
+      if (x < ext[0] && y < ext[1] && z < ext[2]) {
+        v = load_sample(a, ((size_t)(from[2] + z) * a.gdims[1] + (from[1] + y)) * (size_t)a.gdims[0] + (from[0] + x));
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+      }
+      F[i] = v;
+    }
+    if (a.level == 0) store_minmax(a, sh, dense, mn, mx);
+    __syncthreads();
     extrapolate(k, F, ext);
     forward_brick(k, F, a.wfwd, a.coarsest);
-
-    // subband 0 (non-extended interior) into the parent brick
+    // subband 0 (non-extended interior) into the parent level's grid
     // (EncodeBrick, idx2Encode.cpp:303-333; CopyGridExtent, Volume.h:1177-1196)
     if (!a.coarsest) {
-      double* P = a.parents + (size_t)a.parent_of[dense] * k.nvox;
       const DGrid& g = k.sb[0];
-      int lp[3];
-      for (int d = 0; d < 3; ++d) lp[d] = a.brick[d] > 1 ? (b3[d] & 1) : 0;
       const int n = a.sbne[0] * a.sbne[1] * a.sbne[2];
       for (int i = tid; i < n; i += blockDim.x) {
         int x = i % a.sbne[0], y = (i / a.sbne[0]) % a.sbne[1], z = i / (a.sbne[0] * a.sbne[1]);
-        P[vox(k, lp[0] * a.sbne[0] + x, lp[1] * a.sbne[1] + y, lp[2] * a.sbne[2] + z)] =
-            F[vox(k, g.f[0] + g.s[0] * x, g.f[1] + g.s[1] * y, g.f[2] + g.s[2] * z)];
+        const size_t dst = ((size_t)(b3[2] * a.sbne[2] + z) * a.pdims[1] + (b3[1] * a.sbne[1] + y)) * a.pdims[0] +
+                           (b3[0] * a.sbne[0] + x);
+        a.parents[dst] = F[vox(k, g.f[0] + g.s[0] * x, g.f[1] + g.s[1] * y, g.f[2] + g.s[2] * z)];
       }
     }
+    code_subbands(a, F, dense, lds_raw, sh);
+    __syncthreads();
+  }
+}
 
-    for (int sb = 0; sb < k.nsb; ++sb) {
-      const DGrid& g = k.sb[sb];
-      const int nblk = a.blk_n[sb];
-      const uint32_t* blocks = a.blocks + a.blk_off[sb];
-      if (tid == 0) { sh.klo = 0x7fffffff; sh.khi = -0x7fffffff; sh.err = 0; }
-      __syncthreads();
-      // ---- stage A: one wave per block, one coefficient per lane ----
-      for (int b = wave; b < nblk; b += nwaves) {
-        int org[3], bd[3], skip, ci;
-        unpack_block(blocks[b], org, bd, &skip, &ci);
-        if (skip) {
-          if (lane == 0) { S.emax[b] = kSkip; S.np[b] = 0; }
-          continue;
-        }
-        const int nd = (bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1);
-        const int nv = 1 << (2 * nd);
-        const bool act = lane < nv;
-        // gather order: X fastest over the block dims (idx2_BeginFor3)
-        int x = lane % bd[0], y = (lane / bd[0]) % bd[1], z = lane / (bd[0] * bd[1]);
-        double v = 0;
-        if (act)
-          v = F[vox(k, g.f[0] + g.s[0] * (org[0] + x), g.f[1] + g.s[1] * (org[1] + y),
-                    g.f[2] + g.s[2] * (org[2] + z))];
-        // QuantizeF32/F64 (Source/Core/Function.h:58-89)
-        const double maxabs = wave_max(act ? fabs(v) : 0.0);
-        const int emax = k.dtype ? exponent_f64(maxabs) : exponent_f32((float)maxabs);
-        const int prec = prec3[nd];
-        const double scale = pow2d(prec - 1 - emax);
-        i64 q = act ? f64_to_i64(scale * v) : 0;
-        q = zfp_forward_lanes(q, lane, nd);
-        // ForwardShuffle + negabinary (Zfp.h:534-559)
-        int src = nd == 3 ? a.perm.p3[lane] : nd == 2 ? a.perm.p2[lane & 15] : lane;
-        i64 qq = shfl_i64(q, src);
-        const u64 u = ((u64)qq + IDX2G_NBMASK) ^ IDX2G_NBMASK;
-        int stop = max(64 - k.endbp, 57 + a.etol - emax);
-        int np = stop > 63 ? 0 : 64 - stop;
-        for (int p = 0; p < np; ++p) {
-          u64 X = __ballot(act && ((u >> (63 - p)) & 1ull));
-          if (lane == 0) S.pw[b * npl + p] = X;
-        }
-        if (lane == 0) {
-          S.emax[b] = (short)emax;
-          S.np[b] = (unsigned char)np;
-          if (np > 0) {
-            atomicMax(&sh.khi, 63 + emax + kBpBiasDev);
-            atomicMin(&sh.klo, stop + emax + kBpBiasDev);
-          }
-          // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
-          const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
-          if (k.dtype) {
-            uint16_t e = (uint16_t)(emax + 1023);
-            a.exps[row] = (uint8_t)(e & 0xff);
-            a.exps[row + 1] = (uint8_t)(e >> 8);
-          } else {
-            a.exps[row] = (uint8_t)(emax + 127);
-          }
-        }
-      }
-      __syncthreads();
-      // ---- stage B: emitted length of every (block, plane), thread per block ----
-      for (int b = tid; b < nblk; b += blockDim.x) {
-        if (S.emax[b] == kSkip) continue;
-        int org[3], bd[3], skip, ci;
-        unpack_block(blocks[b], org, bd, &skip, &ci);
-        const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
-        int n = 0;
-        for (int p = 0; p < S.np[b]; ++p) S.plen[b * npl + p] = (uint16_t)plane_len(S.pw[b * npl + p], n, nv);
-      }
-      __syncthreads();
-      const int klo = sh.klo, khi = sh.khi;
-      const int nk = khi >= klo ? khi - klo + 1 : 0;
-      if (nk > kMaxKeys) {
-        if (tid == 0) { flag_error(a.counters, 1); a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0}; }
-        __syncthreads();
-        continue;
-      }
-      // ---- stage C: per key, bit offsets of each block's plane (Morton order) ----
-      for (int kk = tid; kk < nk; kk += blockDim.x) {
-        const int key = klo + kk;
-        uint32_t bits = 0;
-        int first = -1;
-        for (int b = 0; b < nblk; ++b) {
-          const int e = S.emax[b];
-          if (e == kSkip) continue;
-          const int p = 63 - (key - kBpBiasDev - e);
-          if (p >= 0 && p < S.np[b]) {
-            S.poff[b * npl + p] = bits;
-            bits += S.plen[b * npl + p];
-            if (first < 0) first = b;
-          }
-        }
-        S.kbits[kk] = bits;
-        S.kfirst[kk] = (short)first;
-      }
-      __syncthreads();
-      // ---- stage D: byte layout of the key streams, arena + entry reservation ----
-      __shared__ unsigned long long s_base, s_ebase;
-      __shared__ uint32_t s_total;
-      if (tid == 0) {
-        uint32_t tot = 0;
-        for (int kk = 0; kk < nk; ++kk) {
-          S.kboff[kk] = tot;
-          tot += (S.kbits[kk] + 7) >> 3;
-        }
-        s_total = tot;
-        const unsigned long long alloc = (tot + 3ull) & ~3ull;
-        s_base = alloc ? atomicAdd(&a.counters[0], alloc) : 0;
-        s_ebase = nk ? atomicAdd(&a.counters[1], (unsigned long long)nk) : 0;
-        const bool over = s_base + alloc > a.arena_cap || s_ebase + nk > a.entry_cap;
-        if (over) flag_error(a.counters, 2);
-        SbHdr h;
-        h.klo = klo;
-        h.nkeys = over ? 0 : nk;
-        h.ebase = s_ebase;
-        a.hdrs[(size_t)dense * k.nsb + sb] = h;
-      }
-      __syncthreads();
-      const unsigned long long base = s_base;
-      const uint32_t total = s_total;
-      if (base + ((total + 3ull) & ~3ull) > a.arena_cap || s_ebase + nk > a.entry_cap) continue;
-      for (int kk = tid; kk < nk; kk += blockDim.x) {
-        SbEntry e;
-        e.off = base + S.kboff[kk];
-        e.size = (S.kbits[kk] + 7) >> 3;
-        e.first_block = (uint16_t)(S.kfirst[kk] < 0 ? 0xffff : S.kfirst[kk]);
-        e.key = (uint16_t)(klo + kk);
-        a.entries[s_ebase + kk] = e;
-      }
-      // ---- stage E: write the streams (LDS staging, else global) ----
-      const uint32_t nwords = (total + 3) >> 2;
-      const bool staged = nwords <= (uint32_t)kStageWords;
-      uint32_t* gwords = (uint32_t*)(a.arena + base);
-      if (staged) {
-        for (uint32_t i = tid; i < nwords; i += blockDim.x) S.stage[i] = 0;
-      } else {
-        for (uint32_t i = tid; i < nwords; i += blockDim.x) gwords[i] = 0;
-        __threadfence();
-      }
-      __syncthreads();
-      for (int b = tid; b < nblk; b += blockDim.x) {
-        const int e = S.emax[b];
-        if (e == kSkip) continue;
-        int org[3], bd[3], skip, ci;
-        unpack_block(blocks[b], org, bd, &skip, &ci);
-        const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
-        int n = 0;
-        for (int p = 0; p < S.np[b]; ++p) {
-          const u64 X = S.pw[b * npl + p];
-          u64 w[3];
-          const int len = plane_emit(X, n, nv, w);
-          plane_len(X, n, nv);  // advance n
-          const int kk = (63 - p) + e + kBpBiasDev - klo;
-          const u64 pos = (u64)S.kboff[kk] * 8 + S.poff[b * npl + p];
-          if (staged) or_bits<true>(S.stage, pos, w, len);
-          else or_bits<false>(gwords, pos, w, len);
-        }
-      }
-      __syncthreads();
-      if (staged)
-        for (uint32_t i = tid; i < nwords; i += blockDim.x) gwords[i] = S.stage[i];
-      __syncthreads();
+// Full 32^3 bricks of a 3D grid: the transform runs on chip (brick32.hpp),
+// the transformed brick is handed to the coding stages through the
+// workgroup's scratch volume.
+__global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restrict__ ap) {
+  const EncArgs& a = *ap;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ EncShared sh;
+  __shared__ double w8[8];
+  b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
+  const BrickConst& k = a.k;
+  if (threadIdx.x < 8) {
+    const int s = threadIdx.x;
+    const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
+    w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.wfwd[s];
+  }
+  const unsigned plane = (unsigned)a.gdims[1] * (unsigned)a.gdims[0];
+  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+    const int dense = a.list ? a.list[bi] : a.b0 + bi;
+    int b3[3], from[3], ext[3];
+    brick_origin(a, dense, b3, from, ext);
+    const b32::Pos p = b32::pos();
+    double c[b32::kN];
+    const unsigned lo = (unsigned)p.y * (unsigned)a.gdims[0] + (unsigned)p.x;
+    // uniform base of the brick + 32-bit lane/z offsets
+    const size_t base = ((size_t)from[2] * a.gdims[1] + from[1]) * (size_t)a.gdims[0] + from[0];
+    if (a.in_dtype) {
+      const double* src = (const double*)a.input + base;
+#pragma unroll
+      for (int z = 0; z < 32; ++z) c[z] = src[lo + (unsigned)z * plane];
+    } else {
+      const float* src = (const float*)a.input + base;
+#pragma unroll
+      for (int z = 0; z < 32; ++z) c[z] = (double)src[lo + (unsigned)z * plane];
     }
+    c[32] = 0.0;
+    if (a.level == 0) {
+      double mn = c[0], mx = c[0];
+#pragma unroll
+      for (int z = 1; z < 32; ++z) {
+        mn = c[z] < mn ? c[z] : mn;
+        mx = c[z] > mx ? c[z] : mx;
+      }
+      store_minmax(a, sh, dense, mn, mx);
+    }
+    __syncthreads();
+    b32::extrapolate(c, S, p);
+    b32::forward(c, S, p);
+    if (!a.coarsest && !(p.x & 1) && !(p.y & 1)) {
+      const unsigned pplane = (unsigned)a.pdims[1] * (unsigned)a.pdims[0];
+      double* dst = a.parents + ((size_t)(b3[2] * 16) * a.pdims[1] + b3[1] * 16) * a.pdims[0] + b3[0] * 16;
+      const unsigned po = (unsigned)(p.y >> 1) * (unsigned)a.pdims[0] + (unsigned)(p.x >> 1);
+#pragma unroll
+      for (int z = 0; z < 16; ++z) dst[po + (unsigned)z * pplane] = c[2 * z];
+    }
+    b32::weigh(c, S, p, w8);
+    double* F = a.scratch + (size_t)blockIdx.x * k.nvox;
+    const unsigned fo = (unsigned)p.y * 33u + (unsigned)p.x;
+#pragma unroll
+    for (int z = 0; z < b32::kN; ++z) F[fo + (unsigned)z * 1089u] = c[z];
+    for (int t = p.tid; t < b32::kN * 32; t += b32::kTh) {
+      const int z = t >> 5, i = t & 31;
+      F[(z * 33 + i) * 33 + 32] = S.ex[t];
+      F[(z * 33 + 32) * 33 + i] = S.ey[t];
+    }
+    if (p.tid < b32::kN) F[(p.tid * 33 + 32) * 33 + 32] = S.cc[p.tid];
+    __syncthreads();
+    code_subbands(a, F, dense, lds_raw, sh);
     __syncthreads();
   }
 }

@@ -30,6 +30,24 @@ static int dalloc(T** p, size_t n) {
   HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
   return IDX2G_OK;
 }
+// Copy a kernel argument block into the next slot of the context's ring and
+// return its device address.  The ring is drained (stream synchronised)
+// before a slot is reused.
+static constexpr size_t kArgSlot = 2048, kArgSlots = 256;
+template <class T>
+static int stage_args(idx2g_ctx* ctx, const T& a, const T** out) {
+  static_assert(sizeof(T) <= kArgSlot, "argument block too large");
+  if (ctx->arg_slot == (int)kArgSlots) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->arg_slot = 0;
+  }
+  const size_t off = (size_t)ctx->arg_slot++ * kArgSlot;
+  memcpy(ctx->h_args + off, &a, sizeof(T));
+  HIPCHK(hipMemcpyAsync(ctx->d_args + off, ctx->h_args + off, sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  *out = (const T*)(ctx->d_args + off);
+  return IDX2G_OK;
+}
+
 #define TRY(x)                  \
   do {                          \
     int rc_ = (x);              \
@@ -95,65 +113,84 @@ struct EncodeBuffers {
   }
 };
 
+// Bricks of a level that the on-chip kernel takes: 3D, brick 32^3, data
+// extent 32 on every axis.
+static bool full_b32(const Layout& L, const int* b3, const int gd[3]) {
+  // the kernel addresses a brick with 32-bit offsets from its origin
+  if ((uint64_t)gd[0] * (uint64_t)gd[1] * 33ull >= (1ull << 32)) return false;
+  for (int d = 0; d < 3; ++d)
+    if (L.brick[d] != 32 || (b3[d] + 1) * 32 > gd[d]) return false;
+  return true;
+}
+
 static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_input, EncodeBuffers& B,
-                             unsigned long long* flags, unsigned long long* l0_bytes) {
+                             unsigned long long* flags, unsigned long long* l0_bytes, int64_t* parent_bytes) {
   const Layout& L = ds->L;
   const BrickConst kc = make_brick_const(L);
   const PermTables pt = make_perm_tables();
   hipStream_t st = ctx->stream;
   const int npl = L.dtype ? 64 : 32;
   const size_t lds = enc_lds_bytes(npl);
+  const size_t lds32 = std::max(lds, sizeof(b32::Lds));
   HIPCHK(hipFuncSetAttribute((const void*)k_encode_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HIPCHK(hipFuncSetAttribute((const void*)k_encode_b32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32));
   HIPCHK(hipMemsetAsync(B.counters, 0, 4 * sizeof(unsigned long long), st));
-  double* bricks_cur = nullptr;  // accumulated bricks of the current level (level > 0)
+  int sbne[3], gd[3];
+  for (int d = 0; d < 3; ++d) {
+    sbne[d] = L.sb[0].dims[d] - (L.sb[0].dims[d] > 1);
+    gd[d] = L.dims[d];
+  }
+  const int grid_max = std::max(1, ctx->ncu);
   double* scratch = nullptr;
-  const int grid0 = std::min(ds->lv[0].nb, ctx->ncu * 4);
-  TRY(dalloc(&scratch, (size_t)grid0 * kc.nvox));
+  TRY(dalloc(&scratch, (size_t)std::max(grid_max, std::min(ds->lv[0].nb, ctx->ncu * 4)) * kc.nvox));
+  const void* level_in = d_input;
+  double* level_grid = nullptr;  // owned grid of the current level (level > 0)
+  *parent_bytes = 0;
   int rc = IDX2G_OK;
   for (int l = 0; l < L.nlevels && !rc; ++l) {
     LevelData& lv = ds->lv[l];
     const bool coarsest = l + 1 == L.nlevels;
     double* parents = nullptr;
-    int* d_parent_of = nullptr;
+    int pd[3] = {1, 1, 1};
     if (!coarsest) {
-      const LevelData& pl = ds->lv[l + 1];
-      TRY(dalloc(&parents, (size_t)pl.nb * kc.nvox));
-      HIPCHK(hipMemsetAsync(parents, 0, (size_t)pl.nb * kc.nvox * sizeof(double), st));
-      std::map<uint64_t, int> pidx;
-      for (int i = 0; i < pl.nb; ++i) pidx[pl.lin[i]] = i;
-      std::vector<int> parent_of(lv.nb);
-      for (int i = 0; i < lv.nb; ++i) {
-        int p3[3];
-        for (int d = 0; d < 3; ++d) p3[d] = lv.xyz[3 * i + d] / L.group[d];
-        parent_of[i] = pidx[L.linear_brick(l + 1, p3)];
+      size_t n = 1;
+      for (int d = 0; d < 3; ++d) {
+        pd[d] = L.nbricks[l][d] * sbne[d];
+        n *= (size_t)pd[d];
       }
-      TRY(dalloc(&d_parent_of, lv.nb));
-      HIPCHK(hipMemcpyAsync(d_parent_of, parent_of.data(), lv.nb * sizeof(int), hipMemcpyHostToDevice, st));
+      TRY(dalloc(&parents, n));
+      if (l == 0) *parent_bytes = (int64_t)(n * sizeof(double));
     }
     TRY(dalloc(&lv.d_xyz, 3 * lv.nb));
     HIPCHK(hipMemcpyAsync(lv.d_xyz, lv.xyz.data(), 3 * lv.nb * sizeof(int), hipMemcpyHostToDevice, st));
     TRY(dalloc(&lv.d_exps, lv.exp_bytes + 16));
     TRY(dalloc(&B.hdrs[l], (size_t)lv.nb * L.nsb));
+    // split the level's bricks between the on-chip kernel and the generic one
+    std::vector<int> full, rest;
+    for (int i = 0; i < lv.nb; ++i) (full_b32(L, &lv.xyz[3 * i], gd) ? full : rest).push_back(i);
+    int* d_lists = nullptr;
+    TRY(dalloc(&d_lists, (size_t)lv.nb));
+    if (!full.empty())
+      HIPCHK(hipMemcpyAsync(d_lists, full.data(), full.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (!rest.empty())
+      HIPCHK(hipMemcpyAsync(d_lists + full.size(), rest.data(), rest.size() * sizeof(int), hipMemcpyHostToDevice, st));
     EncArgs a;
     memset(&a, 0, sizeof a);
     a.k = kc;
     a.perm = pt;
     a.level = l;
     a.coarsest = coarsest;
-    a.input = d_input;
+    a.input = level_in;
+    a.in_dtype = l == 0 ? L.dtype : 1;
     for (int d = 0; d < 3; ++d) {
-      a.dims[d] = L.dims[d];
+      a.gdims[d] = gd[d];
       a.brick[d] = L.brick[d];
-      a.nbricks_prev[d] = l > 0 ? L.nbricks[l - 1][d] : 0;
-      a.sbne[d] = L.sb[0].dims[d] - (L.sb[0].dims[d] > 1);
+      a.sbne[d] = sbne[d];
+      a.pdims[d] = pd[d];
     }
     a.xyz = lv.d_xyz;
-    a.nb = lv.nb;
-    a.b0 = 0;
-    a.bricks = bricks_cur;
     a.scratch = scratch;
     a.parents = parents;
-    a.parent_of = d_parent_of;
     a.blocks = ds->d_blocks;
     for (int s = 0; s < L.nsb; ++s) {
       a.blk_off[s] = ds->blk_off[l][s];
@@ -177,26 +214,37 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
     a.entries = B.entries;
     a.hdrs = B.hdrs[l];
     a.minmax = ds->d_minmax;
-    const int grid = l == 0 ? grid0 : std::min(lv.nb, ctx->ncu * 4);
     if (l == 0) HIPCHK(hipEventRecord(ctx->ev[3], st));
-    hipLaunchKernelGGL(k_encode_level, dim3(grid), dim3(kThreads), lds, st, a);
-    HIPCHK(hipGetLastError());
+    if (!full.empty()) {
+      a.list = d_lists;
+      a.nb = (int)full.size();
+      const EncArgs* da;
+      TRY(stage_args(ctx, a, &da));
+      hipLaunchKernelGGL(k_encode_b32, dim3(std::min(a.nb, grid_max)), dim3(b32::kTh), lds32, st, da);
+      HIPCHK(hipGetLastError());
+    }
+    if (l == 0 && !full.empty()) HIPCHK(hipEventRecord(ctx->ev[4], st));
+    if (!rest.empty()) {
+      a.list = d_lists + full.size();
+      a.nb = (int)rest.size();
+      const EncArgs* da;
+      TRY(stage_args(ctx, a, &da));
+      hipLaunchKernelGGL(k_encode_level, dim3(std::min(a.nb, ctx->ncu * 4)), dim3(kThreads), lds, st, da);
+      HIPCHK(hipGetLastError());
+    }
     if (l == 0) {
-      HIPCHK(hipEventRecord(ctx->ev[4], st));
+      if (full.empty()) HIPCHK(hipEventRecord(ctx->ev[4], st));
       HIPCHK(hipMemcpyAsync(l0_bytes, B.counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     }
-    if (bricks_cur) {
-      HIPCHK(hipStreamSynchronize(st));
-      (void)hipFree(bricks_cur);
-    }
-    bricks_cur = parents;
-    if (d_parent_of) {
-      HIPCHK(hipStreamSynchronize(st));
-      (void)hipFree(d_parent_of);
-    }
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(d_lists);
+    if (level_grid) (void)hipFree(level_grid);
+    level_grid = parents;
+    level_in = parents;
+    for (int d = 0; d < 3; ++d) gd[d] = pd[d];
   }
   HIPCHK(hipStreamSynchronize(st));
-  if (bricks_cur) (void)hipFree(bricks_cur);
+  if (level_grid) (void)hipFree(level_grid);
   (void)hipFree(scratch);
   HIPCHK(hipMemcpy(flags, B.counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   return rc;
@@ -329,8 +377,9 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
     TRY(dalloc(&B.counters, 4));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     unsigned long long flags[3], l0 = 0;
-    rc = run_encode_levels(ctx, ds, d_in, B, flags, &l0);
-    stats.dominant_bytes = (int64_t)(raw + l0);
+    int64_t pbytes = 0;
+    rc = run_encode_levels(ctx, ds, d_in, B, flags, &l0, &pbytes);
+    stats.dominant_bytes = (int64_t)(raw + l0) + pbytes;
     HIPCHK(hipEventRecord(ctx->ev[1], st));
     if (rc) break;
     if (flags[2] & 1) {

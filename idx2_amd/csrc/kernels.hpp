@@ -33,19 +33,20 @@ struct EncArgs {
   BrickConst k;
   PermTables perm;
   int level, coarsest;
-  // level 0 input
+  // input grid of this level: the caller's grid at level 0, the dense f64
+  // grid of subband-0 interiors above
   const void* input;
-  int dims[3], brick[3];
+  int in_dtype;          // 0 f32, 1 f64
+  int gdims[3], brick[3];
   // bricks of this level, dense (linear-id) order
-  const int* xyz;       // [nb][3]
-  int nb, b0;           // process dense bricks [b0, b0+nb)
-  int nbricks_prev[3];  // bricks per axis of the level below (level > 0)
-  double* bricks;       // level > 0: accumulated bricks [dense][nvox] (transformed in place)
-  double* scratch;      // level 0: [gridDim.x][nvox]
-  // parents (non-coarsest)
-  double* parents;      // [dense parent][nvox]
-  const int* parent_of; // dense parent index per dense brick
-  int sbne[3];          // non-extended subband-0 dims
+  const int* xyz;        // [nb_level][3]
+  const int* list;       // dense ids to process, or null for [b0, b0 + nb)
+  int nb, b0;
+  double* scratch;       // [gridDim.x][nvox] (generic kernel: work volume; b32: transformed brick)
+  // next level's grid (non-coarsest): subband-0 interiors at brick * sbne
+  double* parents;
+  int pdims[3];
+  int sbne[3];           // non-extended subband-0 dims
   // block tables
   const uint32_t* blocks;
   int blk_off[8], blk_n[8], ncoded[8];
