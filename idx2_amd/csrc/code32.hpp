@@ -1,0 +1,479 @@
+// zfp block stage and bit-plane coding of one transformed 32^3 brick
+// (EncodeSubbandBlocks, Source/Core/idx2Encode.cpp:141-256; zfp Encode,
+// Source/Core/Zfp.h:770-813), lane-parallel, for the 1024-thread brick
+// workgroup of brick32.hpp.  Per subband:
+//   1. stage the subband's f64 values from the register columns and the LDS
+//      faces into a dense LDS array V;
+//   2. block stage, 4 blocks per wave: "row mode" (one lane per 4-value row)
+//      for the common exponent, quantisation and the zfp lifts (rows along
+//      the first axis in registers, the other axes through a per-wave LDS
+//      transpose buffer), then "coefficient mode" (one lane per coefficient)
+//      for the coefficient order and negabinary, then a 64x64 bit-matrix
+//      transpose across the wave that leaves one bit plane per lane;
+//   3. per bit plane: significance count N by a suffix-max scan over the
+//      planes above it, emitted length by closed form (no bit loop);
+//   4. per key (bit-plane channel): exclusive scan of the lengths over the
+//      blocks in Morton order, one wave per key;
+//   5. one thread per (block, plane) emits its bits into an LDS stream
+//      buffer (global arena when the streams do not fit), then the buffer is
+//      copied into the arena.
+#pragma once
+
+#include "brick32.hpp"
+#include "kernels.hpp"
+
+namespace idx2g {
+namespace b32 {
+
+constexpr int kStreamWords = 4096;  // LDS stream buffer (16 KB), aliases V
+
+struct CodeLds {
+  union {
+    double V[17 * 17 * 17];
+    uint32_t stream[kStreamWords];
+  } v;
+  i64 T[kTh / 64][4][64];         // row-mode transpose buffers, per wave
+  uint32_t lno[kMaxBlocks][64];   // len | N << 8 | bit offset << 15
+  short E[128];                   // block exponent (kSkip: not coded)
+  unsigned char np[128];          // coded planes of the block
+  uint32_t kbits[kMaxKeys];       // bits of each key stream
+  uint32_t kbyte[kMaxKeys];       // byte offset of each key stream
+  short kfirst[kMaxKeys];
+  unsigned char p3[64], p2[16];
+  int klo, khi;
+  unsigned long long base, ebase;
+  uint32_t total;
+  int over;
+};
+
+// ---- wave helpers ----
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
+}
+
+// Value of lane ^ s for a 32-bit quantity, s in {1, 2, 4, 8, 16, 32}.
+template <int S>
+__device__ __forceinline__ uint32_t xlane(uint32_t v, int lane) {
+  if (S == 1) return dpp_u<0xB1>(v);                  // quad_perm [1,0,3,2]
+  if (S == 2) return dpp_u<0x4E>(v);                  // quad_perm [2,3,0,1]
+  // ds_swizzle bit-mask mode: lane i reads lane (i & 0x1f) ^ S of its 32-lane half
+  if (S == 4) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (4 << 10));
+  if (S == 8) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (8 << 10));
+  if (S == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (lane & 16) ? (uint32_t)p[0] : (uint32_t)p[1];
+  }
+  const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return (lane & 32) ? (uint32_t)p[0] : (uint32_t)p[1];
+}
+
+// One butterfly stage of the 64x64 bit transpose: lanes L (bit s of the
+// lane clear) and U = L ^ s exchange the bit columns whose bit s differs.
+template <int S>
+__device__ __forceinline__ u64 tstage(u64 r, int lane) {
+  constexpr u64 M = S == 32 ? 0x00000000FFFFFFFFull : S == 16 ? 0x0000FFFF0000FFFFull
+                  : S == 8 ? 0x00FF00FF00FF00FFull : S == 4 ? 0x0F0F0F0F0F0F0F0Full
+                  : S == 2 ? 0x3333333333333333ull : 0x5555555555555555ull;
+  const uint32_t plo = xlane<S>((uint32_t)r, lane), phi = xlane<S>((uint32_t)(r >> 32), lane);
+  const u64 P = ((u64)phi << 32) | plo;
+  const bool up = lane & S;
+  const u64 t = up ? (P >> S) : (P << S);
+  const u64 keep = up ? ~M : M;
+  return (r & keep) | (t & ~keep);
+}
+
+// Lane j returns sum_i bit_j(r_i) << i (r_i: the value of lane i).
+__device__ __forceinline__ u64 bit_transpose(u64 r, int lane) {
+  r = tstage<32>(r, lane);
+  r = tstage<16>(r, lane);
+  r = tstage<8>(r, lane);
+  r = tstage<4>(r, lane);
+  r = tstage<2>(r, lane);
+  r = tstage<1>(r, lane);
+  return r;
+}
+
+__device__ __forceinline__ int clz64(u64 x) { return x ? __clzll((long long)x) : 64; }
+
+// Emitted length of one plane given N (closed form of the embedded coder,
+// Zfp.h:770-813): verbatim N bits, then one group test per newly
+// significant coefficient.  Returns the length, updates n.
+__device__ __forceinline__ int plane_len_cf(u64 X, int& n, int nv) {
+  if (n >= nv) return nv;
+  const u64 Y = X >> n;
+  if (Y == 0) return n + 1;
+  const int qm = 63 - clz64(X);
+  const int m = __popcll(Y);
+  const int len = qm + 1 + m - (qm == nv - 1) + (qm + 1 < nv);
+  n = qm + 1;
+  return len;
+}
+
+// Emitted bits of one plane (LSB first) as a 128-bit value.
+__device__ __forceinline__ void plane_bits(u64 X, int n, int nv, u64& lo, u64& hi) {
+  lo = n >= 64 ? X : (X & ((1ull << n) - 1));
+  hi = 0;
+  if (n >= nv) return;
+  int pos = n, cur = n;
+  u64 rest = X >> n;
+  for (;;) {
+    if (rest == 0) break;  // trailing 0: nothing to set
+    const int t = __ffsll((long long)rest) - 1;
+    const int q = cur + t;
+    if (pos < 64) lo |= 1ull << pos; else hi |= 1ull << (pos - 64);
+    pos += 1 + t;
+    if (q < nv - 1) {
+      if (pos < 64) lo |= 1ull << pos; else hi |= 1ull << (pos - 64);
+      pos += 1;
+    }
+    cur = q + 1;
+    rest = (t + 1 >= 64) ? 0 : rest >> (t + 1);
+    if (cur >= nv) break;
+  }
+}
+
+// OR a 128-bit little-endian bit string into 32-bit words at bit position pos.
+template <bool kLds>
+__device__ __forceinline__ void or_bits128(uint32_t* words, u64 pos, u64 lo, u64 hi) {
+  const int sh = int(pos & 31);
+  uint32_t* w = words + (pos >> 5);
+  const u64 a0 = lo << sh;
+  const u64 a1 = sh ? ((hi << sh) | (lo >> (64 - sh))) : hi;
+  const uint32_t a2 = sh ? (uint32_t)(hi >> (64 - sh)) : 0u;
+  const uint32_t d[5] = {(uint32_t)a0, (uint32_t)(a0 >> 32), (uint32_t)a1, (uint32_t)(a1 >> 32), a2};
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    if (d[i]) atomicOr(&w[i], d[i]);
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- 1. stage subband sb of the transformed brick into V ----
+template <int HZ>
+__device__ __forceinline__ void stage_cols(const double (&c)[kN], double* V, int i, int j, int dx, int dy, int dz) {
+  double* q = V + j * dx + i;
+#pragma unroll
+  for (int k = 0; k < 17; ++k)
+    if (k < dz) q[k * dx * dy] = c[HZ + 2 * k];
+}
+
+__device__ __forceinline__ void stage_subband(const double (&c)[kN], const Lds& S, double* V, const Pos& p, int hx,
+                                              int hy, int hz) {
+  const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+  if ((p.x & 1) == hx && (p.y & 1) == hy) {
+    if (hz) stage_cols<1>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
+    else stage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
+  }
+  // faces: x = 32 (hx = 0), y = 32 (hy = 0), corner
+  for (int t = p.tid; t < 17 * 16 * 2 + 17; t += kTh) {
+    if (t < 272) {
+      if (hx) continue;
+      const int k = t >> 4, j = t & 15;
+      if (k < dz) V[(k * dy + j) * dx + 16] = S.ex[(hz + 2 * k) * 32 + hy + 2 * j];
+    } else if (t < 544) {
+      if (hy) continue;
+      const int k = (t - 272) >> 4, i = (t - 272) & 15;
+      if (k < dz) V[(k * dy + 16) * dx + i] = S.ey[(hz + 2 * k) * 32 + hx + 2 * i];
+    } else {
+      if (hx || hy) continue;
+      const int k = t - 544;
+      if (k < dz) V[(k * dy + 16) * dx + 16] = S.cc[hz + 2 * k];
+    }
+  }
+}
+
+
+// Coefficient c of block slot qq (coefficient order, then negabinary),
+// ForwardShuffle (Zfp.h:534-559).
+__device__ __forceinline__ u64 coef_u(const i64* Tq, const CodeLds& C, int nd, int c) {
+  const int nv = 1 << (2 * nd);
+  if (c >= nv) return 0;
+  const int src = nd == 3 ? C.p3[c] : nd == 2 ? C.p2[c] : c;
+  return ((u64)Tq[src] + IDX2G_NBMASK) ^ IDX2G_NBMASK;
+}
+
+// Plane mode after the transpose: lane j holds plane word X of block slot
+// `slot` (of the wave's group) and plane pp.  seg: lanes per block (32 for
+// f32 pairs, 64 for f64).
+__device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bool blk_ok, int np, int nd, int pp,
+                                           CodeLds& C, u64* XG) {
+  const int k = lane & (seg - 1);
+  const bool coded = blk_ok && pp < np;
+  const int t = (coded && X) ? 64 - clz64(X) : 0;
+  int incl = t;
+#pragma unroll
+  for (int s = 1; s < 64; s <<= 1) {
+    const int o = __shfl_down(incl, s, 64);
+    if (s < seg && k + s < seg) incl = max(incl, o);
+  }
+  const int nx = __shfl_down(incl, 1, 64);
+  int n = (k + 1 < seg) ? nx : 0;
+  if (coded) {
+    const int n_in = n;
+    const int len = plane_len_cf(X, n, 1 << (2 * nd));
+    C.lno[blk][pp] = (uint32_t)len | ((uint32_t)n_in << 8);
+    XG[blk * 64 + pp] = X;
+  }
+}
+
+// Block stage of one subband staged in C.v.V (dims dx, dy, ...): per block
+// exponent, planes and their (length, N); 4 blocks per wave.
+__device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint32_t* blocks, int nblk, CodeLds& C,
+                            const Pos& p, int dense, u64* XG) {
+  const BrickConst& k = a.k;
+  const int lane = p.lane, q = lane >> 4, r = lane & 15;
+  i64* T = &C.T[p.w][0][0];
+  for (int g = p.w; g * 4 < nblk; g += kTh / 64) {
+    const int b = g * 4 + q;
+    const bool bin = b < nblk;
+    const uint32_t bw = bin ? blocks[b] : (1u << 24);
+    const int o3[3] = {int(bw & 31), int((bw >> 5) & 31), int((bw >> 10) & 31)};
+    const int bd[3] = {int((bw >> 15) & 7), int((bw >> 18) & 7), int((bw >> 21) & 7)};
+    const bool skip = !bin || ((bw >> 24) & 1);
+    const int ci = int(bw >> 25);
+    // V strides of the block's non-unit axes, in order (x before y before z)
+    const bool ux = bd[0] > 1, uy = bd[1] > 1, uz = bd[2] > 1;
+    const int nd = ux + uy + uz;
+    const int dxy = dx * dy;
+    const int s0 = ux ? 1 : (uy ? dx : dxy);
+    const int s1 = ux ? (uy ? dx : dxy) : (uy ? dxy : 0);
+    const int s2 = (ux && uy && uz) ? dxy : 0;
+    const int i1 = r & 3, i2 = r >> 2;
+    const bool rowok = !skip && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
+    const int n0 = nd >= 1 ? 4 : 1;
+    const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
+    double v[4];
+    int emx = -4000;
+#pragma unroll
+    for (int i0 = 0; i0 < 4; ++i0) {
+      const bool ok = rowok && i0 < n0;
+      v[i0] = ok ? C.v.V[vbase + i0 * s0] : 0.0;
+      const int ev = k.dtype ? exponent_f64(fabs(v[i0])) : exponent_f32((float)fabs(v[i0]));
+      if (ok) emx = max(emx, ev);
+    }
+    emx = max(emx, (int)xlane<1>((uint32_t)emx, lane));
+    emx = max(emx, (int)xlane<2>((uint32_t)emx, lane));
+    emx = max(emx, (int)xlane<4>((uint32_t)emx, lane));
+    emx = max(emx, (int)xlane<8>((uint32_t)emx, lane));
+    const int emax = emx;
+    const double scale = pow2d((63 - nd) - 1 - emax);
+    i64 w4[4];
+#pragma unroll
+    for (int i0 = 0; i0 < 4; ++i0) w4[i0] = f64_to_i64(scale * v[i0]);
+    if (nd >= 1) zfp_flift4(w4[0], w4[1], w4[2], w4[3]);
+    i64* Tq = T + q * 64;
+#pragma unroll
+    for (int i0 = 0; i0 < 4; ++i0) Tq[(i1 + 4 * i2) * 4 + i0] = w4[i0];
+    wave_sync_lds();
+    {  // second axis: lane (j0, j2) lifts positions j0 + 4 i + 16 j2
+      const int j0 = r & 3, j2 = r >> 2;
+      i64 y4[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + 16 * j2];
+      if (nd >= 2) {
+        zfp_flift4(y4[0], y4[1], y4[2], y4[3]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + 16 * j2] = y4[i];
+      }
+    }
+    wave_sync_lds();
+    {  // third axis: lane (j0, j1) lifts positions j0 + 4 j1 + 16 i
+      const int j0 = r & 3, j1 = r >> 2;
+      i64 y4[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * j1 + 16 * i];
+      if (nd == 3) {
+        zfp_flift4(y4[0], y4[1], y4[2], y4[3]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * j1 + 16 * i] = y4[i];
+      }
+    }
+    wave_sync_lds();
+    // block scalars
+    const int stop = max(64 - k.endbp, 57 + a.etol - emax);
+    const int np = skip ? 0 : (stop > 63 ? 0 : 64 - stop);
+    if (r == 0 && bin) {
+      C.E[b] = skip ? kSkip : (short)emax;
+      C.np[b] = (unsigned char)np;
+      if (!skip) {
+        // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
+        const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
+        if (k.dtype) {
+          const uint16_t e16 = (uint16_t)(emax + 1023);
+          a.exps[row] = (uint8_t)(e16 & 0xff);
+          a.exps[row + 1] = (uint8_t)(e16 >> 8);
+        } else {
+          a.exps[row] = (uint8_t)(emax + 127);
+        }
+        if (np > 0) {
+          atomicMax(&C.khi, 63 + emax + kBpBiasDev);
+          atomicMin(&C.klo, stop + emax + kBpBiasDev);
+        }
+      }
+    }
+    // coefficient mode + bit transpose + plane mode
+    const int nd_s[4] = {__shfl(nd, 0, 64), __shfl(nd, 16, 64), __shfl(nd, 32, 64), __shfl(nd, 48, 64)};
+    const int np_s[4] = {__shfl(np, 0, 64), __shfl(np, 16, 64), __shfl(np, 32, 64), __shfl(np, 48, 64)};
+    if (k.dtype == 0) {
+#pragma unroll
+      for (int pr = 0; pr < 2; ++pr) {
+        const u64 uA = coef_u(T + (2 * pr) * 64, C, nd_s[2 * pr], lane);
+        const u64 uB = coef_u(T + (2 * pr + 1) * 64, C, nd_s[2 * pr + 1], lane);
+        const u64 R = (uA >> 32) | (uB & 0xFFFFFFFF00000000ull);
+        const u64 X = bit_transpose(R, lane);
+        const int slot = 2 * pr + (lane >> 5);
+        const int blk = g * 4 + slot;
+        const int nds = (lane >> 5) ? nd_s[2 * pr + 1] : nd_s[2 * pr];
+        const int nps = (lane >> 5) ? np_s[2 * pr + 1] : np_s[2 * pr];
+        plane_mode(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG);
+      }
+    } else {
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const u64 u = coef_u(T + qq * 64, C, nd_s[qq], lane);
+        const u64 X = bit_transpose(u, lane);
+        const int blk = g * 4 + qq;
+        plane_mode(X, lane, 64, blk, blk < nblk, np_s[qq], nd_s[qq], 63 - lane, C, XG);
+      }
+    }
+    wave_sync_lds();
+  }
+}
+
+// Code every subband of the brick held in c (register columns) and S
+// (faces); writes exponent rows, streams into the arena, entries, headers.
+__device__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, CodeLds& C, const Pos& p, int dense,
+                           u64* XG) {
+  const BrickConst& k = a.k;
+  const int npl = k.dtype ? 64 : 32;
+  const int lane = p.lane, q = lane >> 4, r = lane & 15;
+  i64* T = &C.T[p.w][0][0];
+  if (p.tid < 64) C.p3[p.tid] = a.perm.p3[p.tid];
+  if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
+  for (int sb = 0; sb < 8; ++sb) {
+    const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
+    const int dx = 17 - hx, dy = 17 - hy;
+    const int nblk = a.blk_n[sb];
+    const uint32_t* blocks = a.blocks + a.blk_off[sb];
+    stage_subband(c, S, C.v.V, p, hx, hy, hz);
+    if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; C.over = 0; }
+    __syncthreads();
+    block_stage(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
+    __syncthreads();
+    // ---- 4. per-key offsets over the blocks (Morton order) ----
+    const int klo = C.klo, khi = C.khi;
+    const int nk = khi >= klo ? khi - klo + 1 : 0;
+    if (nk > kMaxKeys) {
+      if (p.tid == 0) {
+        atomicOr(&a.counters[2], 1ull);
+        a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0};
+      }
+      __syncthreads();
+      continue;
+    }
+    for (int kk = p.w; kk < nk; kk += kTh / 64) {
+      const int key = klo + kk;
+      uint32_t carry = 0;
+      int first = -1;
+      for (int c0 = 0; c0 < nblk; c0 += 64) {
+        const int b = c0 + lane;
+        const int e = b < nblk ? C.E[b] : kSkip;
+        const int pp = 63 - (key - kBpBiasDev - e);
+        const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
+        const uint32_t len = coded ? (C.lno[b][pp] & 0xff) : 0u;
+        uint32_t incl = len;
+#pragma unroll
+        for (int s = 1; s < 64; s <<= 1) {
+          const uint32_t o = __shfl_up(incl, s, 64);
+          if (lane >= s) incl += o;
+        }
+        if (coded) C.lno[b][pp] |= (incl - len + carry) << 15;
+        const u64 m = __ballot(coded);
+        if (first < 0 && m) first = c0 + __ffsll((long long)m) - 1;
+        carry += __shfl(incl, 63, 64);
+      }
+      if (lane == 0) {
+        C.kbits[kk] = carry;
+        C.kfirst[kk] = (short)first;
+      }
+    }
+    __syncthreads();
+    // byte layout of the key streams + arena reservation (wave 0)
+    if (p.w == 0) {
+      uint32_t carry = 0;
+      for (int c0 = 0; c0 < nk; c0 += 64) {
+        const int kk = c0 + lane;
+        const uint32_t by = kk < nk ? (C.kbits[kk] + 7) >> 3 : 0u;
+        uint32_t incl = by;
+#pragma unroll
+        for (int s = 1; s < 64; s <<= 1) {
+          const uint32_t o = __shfl_up(incl, s, 64);
+          if (lane >= s) incl += o;
+        }
+        if (kk < nk) C.kbyte[kk] = incl - by + carry;
+        carry += __shfl(incl, 63, 64);
+      }
+      if (lane == 0) {
+        const unsigned long long alloc = (carry + 3ull) & ~3ull;
+        const unsigned long long bse = alloc ? atomicAdd(&a.counters[0], alloc) : 0;
+        const unsigned long long ebs = nk ? atomicAdd(&a.counters[1], (unsigned long long)nk) : 0;
+        const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
+        if (over) atomicOr(&a.counters[2], 2ull);
+        C.base = bse;
+        C.ebase = ebs;
+        C.total = carry;
+        C.over = over;
+        SbHdr h;
+        h.klo = klo;
+        h.nkeys = over ? 0 : nk;
+        h.ebase = ebs;
+        a.hdrs[(size_t)dense * k.nsb + sb] = h;
+      }
+    }
+    __syncthreads();
+    if (C.over) continue;
+    const unsigned long long base = C.base;
+    const uint32_t nwords = (C.total + 3) >> 2;
+    const bool staged = nwords <= (uint32_t)kStreamWords;
+    uint32_t* gwords = (uint32_t*)(a.arena + base);
+    for (int kk = p.tid; kk < nk; kk += kTh) {
+      SbEntry e;
+      e.off = base + C.kbyte[kk];
+      e.size = (C.kbits[kk] + 7) >> 3;
+      e.first_block = (uint16_t)(C.kfirst[kk] < 0 ? 0xffff : C.kfirst[kk]);
+      e.key = (uint16_t)(klo + kk);
+      a.entries[C.ebase + kk] = e;
+    }
+    // ---- 5. emission ----
+    uint32_t* words = staged ? C.v.stream : gwords;
+    for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
+    if (!staged) __threadfence();
+    __syncthreads();
+    for (int t = p.tid; t < nblk * npl; t += kTh) {
+      const int b = t / npl, pp = t - b * npl;
+      const int e = C.E[b];
+      if (e == kSkip || pp >= C.np[b]) continue;
+      const uint32_t bw = blocks[b];
+      const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
+      const uint32_t lno = C.lno[b][pp];
+      u64 lo, hi;
+      plane_bits(XG[b * 64 + pp], int((lno >> 8) & 0x7f), 1 << (2 * nd), lo, hi);
+      const int kk = (63 - pp) + e + kBpBiasDev - klo;
+      const u64 pos = (u64)C.kbyte[kk] * 8 + (lno >> 15);
+      if (staged) or_bits128<true>(words, pos, lo, hi);
+      else or_bits128<false>(words, pos, lo, hi);
+    }
+    __syncthreads();
+    if (staged)
+      for (uint32_t i = p.tid; i < nwords; i += kTh) gwords[i] = words[i];
+    __syncthreads();
+  }
+}
+
+}  // namespace b32
+}  // namespace idx2g

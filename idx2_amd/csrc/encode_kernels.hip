@@ -9,6 +9,7 @@
 
 #include "kernels.hpp"
 #include "brick32.hpp"
+#include "code32.hpp"
 
 namespace idx2g {
 
@@ -395,18 +396,8 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
       for (int z = 0; z < 16; ++z) dst[po + (unsigned)z * pplane] = c[2 * z];
     }
     b32::weigh(c, S, p, w8);
-    double* F = a.scratch + (size_t)blockIdx.x * k.nvox;
-    const unsigned fo = (unsigned)p.y * 33u + (unsigned)p.x;
-#pragma unroll
-    for (int z = 0; z < b32::kN; ++z) F[fo + (unsigned)z * 1089u] = c[z];
-    for (int t = p.tid; t < b32::kN * 32; t += b32::kTh) {
-      const int z = t >> 5, i = t & 31;
-      F[(z * 33 + i) * 33 + 32] = S.ex[t];
-      F[(z * 33 + 32) * 33 + i] = S.ey[t];
-    }
-    if (p.tid < b32::kN) F[(p.tid * 33 + 32) * 33 + 32] = S.cc[p.tid];
-    __syncthreads();
-    code_subbands(a, F, dense, lds_raw, sh);
+    b32::code_brick(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
+                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * k.nvox));
     __syncthreads();
   }
 }

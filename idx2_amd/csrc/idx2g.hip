@@ -131,7 +131,7 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
   hipStream_t st = ctx->stream;
   const int npl = L.dtype ? 64 : 32;
   const size_t lds = enc_lds_bytes(npl);
-  const size_t lds32 = std::max(lds, sizeof(b32::Lds));
+  const size_t lds32 = std::max(sizeof(b32::Lds), offsetof(b32::Lds, u) + sizeof(b32::CodeLds));
   HIPCHK(hipFuncSetAttribute((const void*)k_encode_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   HIPCHK(hipFuncSetAttribute((const void*)k_encode_b32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32));
   HIPCHK(hipMemsetAsync(B.counters, 0, 4 * sizeof(unsigned long long), st));
