@@ -26,6 +26,14 @@ namespace idx2g {
 namespace b32 {
 
 constexpr int kStreamWords = 4096;  // LDS stream buffer (16 KB), aliases V
+constexpr unsigned long long kArenaChunk = 256ull << 10;  // arena bytes a workgroup takes at a time
+constexpr unsigned long long kEntryChunk = 2048;          // stream entries a workgroup takes at a time
+
+// A workgroup's private slices of the level arena and entry table, refilled
+// with one global atomic when exhausted (no atomic round trip per subband).
+struct ArenaCursor {
+  unsigned long long abase, aleft, ebase, eleft;
+};
 
 struct CodeLds {
   union {
@@ -33,13 +41,16 @@ struct CodeLds {
     uint32_t stream[kStreamWords];
   } v;
   i64 T[kTh / 64][4][64];         // row-mode transpose buffers, per wave
-  uint32_t lno[kMaxBlocks][64];   // len | N << 8 | bit offset << 15
+  // per (block, plane): len | N << 8 | bit offset << 15, stride npl; for
+  // f32 the plane words follow (f64 plane words go to global scratch)
+  uint32_t pool[16384];
   short E[128];                   // block exponent (kSkip: not coded)
   unsigned char np[128];          // coded planes of the block
   uint32_t kbits[kMaxKeys];       // bits of each key stream
   uint32_t kbyte[kMaxKeys];       // byte offset of each key stream
   short kfirst[kMaxKeys];
   unsigned char p3[64], p2[16];
+  uint16_t e8[256];               // byte -> bits with a 1 inserted after every 1
   int klo, khi;
   unsigned long long base, ebase;
   uint32_t total;
@@ -53,19 +64,71 @@ __device__ __forceinline__ uint32_t dpp_u(uint32_t v) {
 }
 
 // Value of lane ^ s for a 32-bit quantity, s in {1, 2, 4, 8, 16, 32}.
+// All DPP moves run with the full wave active (a DPP read of a lane that is
+// switched off returns 0), hence the asm that keeps both row shifts
+// outside of any branch.
 template <int S>
 __device__ __forceinline__ uint32_t xlane(uint32_t v, int lane) {
   if (S == 1) return dpp_u<0xB1>(v);                  // quad_perm [1,0,3,2]
   if (S == 2) return dpp_u<0x4E>(v);                  // quad_perm [2,3,0,1]
-  // ds_swizzle bit-mask mode: lane i reads lane (i & 0x1f) ^ S of its 32-lane half
-  if (S == 4) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (4 << 10));
-  if (S == 8) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (8 << 10));
+  if (S == 4 || S == 8) {
+    uint32_t up = dpp_u<0x110 + S>(v), dn = dpp_u<0x100 + S>(v);  // row_shr:S, row_shl:S
+    asm volatile("" : "+v"(up), "+v"(dn));
+    return (lane & S) ? up : dn;
+  }
   if (S == 16) {
     const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
     return (lane & 16) ? (uint32_t)p[0] : (uint32_t)p[1];
   }
   const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
   return (lane & 32) ? (uint32_t)p[0] : (uint32_t)p[1];
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v) { return __builtin_amdgcn_mov_dpp(v, CTRL, 0xf, 0xf, true); }
+
+// Max over each 16-lane row, in every lane of the row.
+__device__ __forceinline__ int row_max(int v) {
+  v = max(v, dppi<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = max(v, dppi<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = max(v, dppi<0x141>(v));  // row_half_mirror
+  v = max(v, dppi<0x140>(v));  // row_mirror
+  return v;
+}
+
+// Inclusive prefix sum over the 64 lanes.
+__device__ __forceinline__ uint32_t wave_scan_add(uint32_t v, int lane) {
+  v += (uint32_t)dppi<0x111>((int)v);  // row_shr:1 (0 shifted in)
+  v += (uint32_t)dppi<0x112>((int)v);
+  v += (uint32_t)dppi<0x114>((int)v);
+  v += (uint32_t)dppi<0x118>((int)v);
+  const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  const uint32_t t2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  const int row = lane >> 4;
+  v += row == 0 ? 0u : row == 1 ? t0 : row == 2 ? t0 + t1 : t0 + t1 + t2;
+  return v;
+}
+
+// Exclusive suffix max (over the higher lanes) inside segments of seg lanes
+// (32 or 64) of non-negative values.
+__device__ __forceinline__ int seg_suffix_max_excl(int v, int lane, int seg) {
+  v = max(v, dppi<0x101>(v));  // row_shl:1 (0 shifted in)
+  v = max(v, dppi<0x102>(v));
+  v = max(v, dppi<0x104>(v));
+  v = max(v, dppi<0x108>(v));
+  const int r1 = __builtin_amdgcn_readlane(v, 16), r2 = __builtin_amdgcn_readlane(v, 32),
+            r3 = __builtin_amdgcn_readlane(v, 48);
+  const int row = lane >> 4;
+  int add;
+  if (seg == 32) add = row == 0 ? r1 : row == 2 ? r3 : 0;
+  else add = row == 0 ? max(r1, max(r2, r3)) : row == 1 ? max(r2, r3) : row == 2 ? r3 : 0;
+  v = max(v, add);
+  int nx = dppi<0x101>(v);
+  const int s1 = __builtin_amdgcn_readlane(v, 16), s2 = __builtin_amdgcn_readlane(v, 32),
+            s3 = __builtin_amdgcn_readlane(v, 48);
+  if ((lane & 15) == 15) nx = row == 0 ? s1 : row == 1 ? s2 : row == 2 ? s3 : 0;
+  return ((lane & (seg - 1)) + 1 < seg) ? nx : 0;
 }
 
 // One butterfly stage of the 64x64 bit transpose: lanes L (bit s of the
@@ -110,27 +173,50 @@ __device__ __forceinline__ int plane_len_cf(u64 X, int& n, int nv) {
   return len;
 }
 
-// Emitted bits of one plane (LSB first) as a 128-bit value.
-__device__ __forceinline__ void plane_bits(u64 X, int n, int nv, u64& lo, u64& hi) {
+// Emitted bits of one plane (LSB first) as a 128-bit value, without a bit
+// loop.  After the N verbatim bits, the group tests turn the remaining bits
+// Y = X >> N, read up to its last one t_m, into Y with a 1 inserted before
+// every run "0..01"; with E() = "a 1 inserted after every 1":
+//   tail = 1 | E(Y below t_m) << 1 | 1 << (t_m + m)   (m ones in Y)
+// where the last 1 is dropped when t_m is the block's last coefficient
+// (it is implied) and a 0 follows when coefficients remain.  E() is
+// assembled from a 256-entry byte table in a three-level tree.
+__device__ __forceinline__ void plane_bits(u64 X, int n, int nv, const uint16_t* e8, u64& lo, u64& hi) {
   lo = n >= 64 ? X : (X & ((1ull << n) - 1));
   hi = 0;
   if (n >= nv) return;
-  int pos = n, cur = n;
-  u64 rest = X >> n;
-  for (;;) {
-    if (rest == 0) break;  // trailing 0: nothing to set
-    const int t = __ffsll((long long)rest) - 1;
-    const int q = cur + t;
-    if (pos < 64) lo |= 1ull << pos; else hi |= 1ull << (pos - 64);
-    pos += 1 + t;
-    if (q < nv - 1) {
-      if (pos < 64) lo |= 1ull << pos; else hi |= 1ull << (pos - 64);
-      pos += 1;
-    }
-    cur = q + 1;
-    rest = (t + 1 >= 64) ? 0 : rest >> (t + 1);
-    if (cur >= nv) break;
+  const u64 Y = X >> n;
+  if (Y == 0) return;
+  const int tm = 63 - clz64(Y), m = __popcll(Y);
+  const u64 Yl = tm ? (Y & ((1ull << tm) - 1)) : 0ull;
+  uint32_t p8[8], l8[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t byte = uint32_t(Yl >> (8 * i)) & 0xffu;
+    p8[i] = e8[byte];
+    l8[i] = 8u + __popc(byte);
   }
+  u64 q4[4];
+  uint32_t L4[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    q4[j] = (u64)p8[2 * j] | ((u64)p8[2 * j + 1] << l8[2 * j]);
+    L4[j] = l8[2 * j] + l8[2 * j + 1];
+  }
+  const u64 r0 = q4[0] | (q4[1] << L4[0]), r1 = q4[2] | (q4[3] << L4[2]);
+  const uint32_t R0 = L4[0] + L4[1];  // 16..64
+  // E = r0 | r1 << R0 as 128 bits
+  u64 elo = r0 | (R0 < 64 ? (r1 << R0) : 0ull);
+  u64 ehi = R0 < 64 ? (r1 >> (64 - R0)) : r1;
+  // tail = 1 | E << 1 | value bit of the last one
+  u64 tlo = 1ull | (elo << 1), thi = (ehi << 1) | (elo >> 63);
+  if (n + tm != nv - 1) {
+    const int vb = tm + m;  // <= 127
+    if (vb < 64) tlo |= 1ull << vb; else thi |= 1ull << (vb - 64);
+  }
+  // out = verbatim | tail << n (n < 64)
+  lo |= tlo << n;
+  hi = n ? ((thi << n) | (tlo >> (64 - n))) : thi;
 }
 
 // OR a 128-bit little-endian bit string into 32-bit words at bit position pos.
@@ -188,6 +274,11 @@ __device__ __forceinline__ void stage_subband(const double (&c)[kN], const Lds& 
 }
 
 
+__device__ __forceinline__ uint32_t& lno_at(CodeLds& C, int npl, int b, int pp) { return C.pool[b * npl + pp]; }
+__device__ __forceinline__ u64* plane_at(CodeLds& C, u64* XG, int npl, int b, int pp) {
+  return npl == 32 ? reinterpret_cast<u64*>(C.pool + kMaxBlocks * 32) + b * 32 + pp : XG + b * 64 + pp;
+}
+
 // Coefficient c of block slot qq (coefficient order, then negabinary),
 // ForwardShuffle (Zfp.h:534-559).
 __device__ __forceinline__ u64 coef_u(const i64* Tq, const CodeLds& C, int nd, int c) {
@@ -201,29 +292,23 @@ __device__ __forceinline__ u64 coef_u(const i64* Tq, const CodeLds& C, int nd, i
 // `slot` (of the wave's group) and plane pp.  seg: lanes per block (32 for
 // f32 pairs, 64 for f64).
 __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bool blk_ok, int np, int nd, int pp,
-                                           CodeLds& C, u64* XG) {
+                                           CodeLds& C, u64* XG, int npl) {
   const int k = lane & (seg - 1);
   const bool coded = blk_ok && pp < np;
   const int t = (coded && X) ? 64 - clz64(X) : 0;
-  int incl = t;
-#pragma unroll
-  for (int s = 1; s < 64; s <<= 1) {
-    const int o = __shfl_down(incl, s, 64);
-    if (s < seg && k + s < seg) incl = max(incl, o);
-  }
-  const int nx = __shfl_down(incl, 1, 64);
-  int n = (k + 1 < seg) ? nx : 0;
+  int n = seg_suffix_max_excl(t, lane, seg);
+  (void)k;
   if (coded) {
     const int n_in = n;
     const int len = plane_len_cf(X, n, 1 << (2 * nd));
-    C.lno[blk][pp] = (uint32_t)len | ((uint32_t)n_in << 8);
-    XG[blk * 64 + pp] = X;
+    lno_at(C, npl, blk, pp) = (uint32_t)len | ((uint32_t)n_in << 8);
+    *plane_at(C, XG, npl, blk, pp) = X;
   }
 }
 
 // Block stage of one subband staged in C.v.V (dims dx, dy, ...): per block
 // exponent, planes and their (length, N); 4 blocks per wave.
-__device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint32_t* blocks, int nblk, CodeLds& C,
+__device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint32_t* blocks, int nblk, CodeLds& C,
                             const Pos& p, int dense, u64* XG) {
   const BrickConst& k = a.k;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
@@ -256,10 +341,7 @@ __device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint
       const int ev = k.dtype ? exponent_f64(fabs(v[i0])) : exponent_f32((float)fabs(v[i0]));
       if (ok) emx = max(emx, ev);
     }
-    emx = max(emx, (int)xlane<1>((uint32_t)emx, lane));
-    emx = max(emx, (int)xlane<2>((uint32_t)emx, lane));
-    emx = max(emx, (int)xlane<4>((uint32_t)emx, lane));
-    emx = max(emx, (int)xlane<8>((uint32_t)emx, lane));
+    emx = row_max(emx);
     const int emax = emx;
     const double scale = pow2d((63 - nd) - 1 - emax);
     i64 w4[4];
@@ -317,8 +399,10 @@ __device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint
       }
     }
     // coefficient mode + bit transpose + plane mode
-    const int nd_s[4] = {__shfl(nd, 0, 64), __shfl(nd, 16, 64), __shfl(nd, 32, 64), __shfl(nd, 48, 64)};
-    const int np_s[4] = {__shfl(np, 0, 64), __shfl(np, 16, 64), __shfl(np, 32, 64), __shfl(np, 48, 64)};
+    const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
+                         __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
+    const int np_s[4] = {__builtin_amdgcn_readlane(np, 0), __builtin_amdgcn_readlane(np, 16),
+                         __builtin_amdgcn_readlane(np, 32), __builtin_amdgcn_readlane(np, 48)};
     if (k.dtype == 0) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
@@ -330,7 +414,7 @@ __device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint
         const int blk = g * 4 + slot;
         const int nds = (lane >> 5) ? nd_s[2 * pr + 1] : nd_s[2 * pr];
         const int nps = (lane >> 5) ? np_s[2 * pr + 1] : np_s[2 * pr];
-        plane_mode(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG);
+        plane_mode(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG, 32);
       }
     } else {
 #pragma unroll
@@ -338,7 +422,7 @@ __device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint
         const u64 u = coef_u(T + qq * 64, C, nd_s[qq], lane);
         const u64 X = bit_transpose(u, lane);
         const int blk = g * 4 + qq;
-        plane_mode(X, lane, 64, blk, blk < nblk, np_s[qq], nd_s[qq], 63 - lane, C, XG);
+        plane_mode(X, lane, 64, blk, blk < nblk, np_s[qq], nd_s[qq], 63 - lane, C, XG, 64);
       }
     }
     wave_sync_lds();
@@ -347,19 +431,28 @@ __device__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint
 
 // Code every subband of the brick held in c (register columns) and S
 // (faces); writes exponent rows, streams into the arena, entries, headers.
-__device__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, CodeLds& C, const Pos& p, int dense,
-                           u64* XG) {
+__device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, CodeLds& C, const Pos& p,
+                                           int dense, u64* XG, const uint32_t* s_blocks, ArenaCursor& cur) {
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   i64* T = &C.T[p.w][0][0];
   if (p.tid < 64) C.p3[p.tid] = a.perm.p3[p.tid];
   if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
+  if (p.tid < 256) {
+    uint32_t e = 0;
+    int o = 0;
+    for (int i = 0; i < 8; ++i) {
+      if ((p.tid >> i) & 1) { e |= 3u << o; o += 2; }
+      else o += 1;
+    }
+    C.e8[p.tid] = (uint16_t)e;
+  }
   for (int sb = 0; sb < 8; ++sb) {
     const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
     const int dx = 17 - hx, dy = 17 - hy;
     const int nblk = a.blk_n[sb];
-    const uint32_t* blocks = a.blocks + a.blk_off[sb];
+    const uint32_t* blocks = s_blocks + sb * 128;
     stage_subband(c, S, C.v.V, p, hx, hy, hz);
     if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; C.over = 0; }
     __syncthreads();
@@ -385,20 +478,15 @@ __device__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, Code
         const int e = b < nblk ? C.E[b] : kSkip;
         const int pp = 63 - (key - kBpBiasDev - e);
         const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
-        const uint32_t len = coded ? (C.lno[b][pp] & 0xff) : 0u;
-        uint32_t incl = len;
-#pragma unroll
-        for (int s = 1; s < 64; s <<= 1) {
-          const uint32_t o = __shfl_up(incl, s, 64);
-          if (lane >= s) incl += o;
-        }
-        if (coded) C.lno[b][pp] |= (incl - len + carry) << 15;
+        const uint32_t len = coded ? (lno_at(C, npl, b, pp) & 0xff) : 0u;
+        const uint32_t incl = wave_scan_add(len, lane);
+        if (coded) lno_at(C, npl, b, pp) |= (incl - len + carry) << 15;
         const u64 m = __ballot(coded);
         if (first < 0 && m) first = c0 + __ffsll((long long)m) - 1;
-        carry += __shfl(incl, 63, 64);
-      }
-      if (lane == 0) {
-        C.kbits[kk] = carry;
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+        }
+        if (lane == 0) {
+          C.kbits[kk] = carry;
         C.kfirst[kk] = (short)first;
       }
     }
@@ -409,19 +497,28 @@ __device__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, Code
       for (int c0 = 0; c0 < nk; c0 += 64) {
         const int kk = c0 + lane;
         const uint32_t by = kk < nk ? (C.kbits[kk] + 7) >> 3 : 0u;
-        uint32_t incl = by;
-#pragma unroll
-        for (int s = 1; s < 64; s <<= 1) {
-          const uint32_t o = __shfl_up(incl, s, 64);
-          if (lane >= s) incl += o;
-        }
+        const uint32_t incl = wave_scan_add(by, lane);
         if (kk < nk) C.kbyte[kk] = incl - by + carry;
-        carry += __shfl(incl, 63, 64);
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
       }
       if (lane == 0) {
         const unsigned long long alloc = (carry + 3ull) & ~3ull;
-        const unsigned long long bse = alloc ? atomicAdd(&a.counters[0], alloc) : 0;
-        const unsigned long long ebs = nk ? atomicAdd(&a.counters[1], (unsigned long long)nk) : 0;
+        if (alloc > cur.aleft) {
+          const unsigned long long take = alloc > kArenaChunk ? alloc : kArenaChunk;
+          cur.abase = atomicAdd(&a.counters[0], take);
+          cur.aleft = take;
+        }
+        if ((unsigned long long)nk > cur.eleft) {
+          const unsigned long long take = (unsigned long long)nk > kEntryChunk ? (unsigned long long)nk : kEntryChunk;
+          cur.ebase = atomicAdd(&a.counters[1], take);
+          cur.eleft = take;
+        }
+        const unsigned long long bse = cur.abase, ebs = cur.ebase;
+        cur.abase += alloc;
+        cur.aleft -= alloc;
+        cur.ebase += nk;
+        cur.eleft -= nk;
+        if (carry) atomicAdd(&a.counters[3], (unsigned long long)carry);
         const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
         if (over) atomicOr(&a.counters[2], 2ull);
         C.base = bse;
@@ -460,9 +557,9 @@ __device__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, Code
       if (e == kSkip || pp >= C.np[b]) continue;
       const uint32_t bw = blocks[b];
       const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
-      const uint32_t lno = C.lno[b][pp];
+      const uint32_t lno = lno_at(C, npl, b, pp);
       u64 lo, hi;
-      plane_bits(XG[b * 64 + pp], int((lno >> 8) & 0x7f), 1 << (2 * nd), lo, hi);
+      plane_bits(*plane_at(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
       const int kk = (63 - pp) + e + kBpBiasDev - klo;
       const u64 pos = (u64)C.kbyte[kk] * 8 + (lno >> 15);
       if (staged) or_bits128<true>(words, pos, lo, hi);

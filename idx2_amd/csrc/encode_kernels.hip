@@ -208,6 +208,7 @@ __device__ void code_subbands(const EncArgs& a, const double* F, int dense, unsi
       sh.total = tot;
       const unsigned long long alloc = (tot + 3ull) & ~3ull;
       sh.base = alloc ? atomicAdd(&a.counters[0], alloc) : 0;
+      if (tot) atomicAdd(&a.counters[3], (unsigned long long)tot);
       sh.ebase = nk ? atomicAdd(&a.counters[1], (unsigned long long)nk) : 0;
       const bool over = sh.base + alloc > a.arena_cap || sh.ebase + nk > a.entry_cap;
       if (over) flag_error(a.counters, 2);
@@ -349,7 +350,14 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ EncShared sh;
   __shared__ double w8[8];
+  __shared__ uint32_t s_blocks[8 * 128];
+  __shared__ b32::ArenaCursor cur;
   b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
+  for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
+    const int s = i >> 7, b = i & 127;
+    s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
+  }
+  if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0};
   const BrickConst& k = a.k;
   if (threadIdx.x < 8) {
     const int s = threadIdx.x;
@@ -397,7 +405,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
     }
     b32::weigh(c, S, p, w8);
     b32::code_brick(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
-                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * k.nvox));
+                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * k.nvox), s_blocks, cur);
     __syncthreads();
   }
 }

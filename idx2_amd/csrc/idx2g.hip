@@ -234,7 +234,8 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
     }
     if (l == 0) {
       if (full.empty()) HIPCHK(hipEventRecord(ctx->ev[4], st));
-      HIPCHK(hipMemcpyAsync(l0_bytes, B.counters, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+      // stream bytes written by level 0 (counter 3 excludes workgroup slack)
+      HIPCHK(hipMemcpyAsync(l0_bytes, B.counters + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
     (void)hipFree(d_lists);
@@ -367,8 +368,11 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
   int nbt = 0;
   for (int l = 0; l < L.nlevels; ++l) nbt += ds->lv[l].nb;
   EncodeBuffers B;
-  B.arena_cap = std::max<unsigned long long>(64ull << 20, (unsigned long long)(raw * 1.5));
-  B.entry_cap = (unsigned long long)nbt * L.nsb * 48 + 4096;
+  // workgroups take arena/entry slices of b32::kArenaChunk / kEntryChunk:
+  // leave room for one partly used slice per workgroup and level
+  const unsigned long long slack = (unsigned long long)ctx->ncu * L.nlevels;
+  B.arena_cap = std::max<unsigned long long>(64ull << 20, (unsigned long long)(raw * 1.5)) + slack * b32::kArenaChunk;
+  B.entry_cap = (unsigned long long)nbt * L.nsb * 48 + 4096 + slack * b32::kEntryChunk;
   hipStream_t st = ctx->stream;
   idx2g_stats stats{};
   for (int attempt = 0; attempt < 4; ++attempt) {
