@@ -1,0 +1,369 @@
+// Decode of one full 32^3 brick by a 1024-thread workgroup, for gfx950
+// (DecodeBrick, Source/Core/idx2Decode.cpp:238-321; DecodeSubband,
+// idx2Decode.cpp:85-235; zfp Decode, Source/Core/Zfp.h:988-1059):
+//   1. the brick's bit-plane streams (all subbands, all keys) are copied
+//      from the chunk blob into LDS when they fit;
+//   2. eight waves walk the streams, one subband each: lane j owns key
+//      khi - j and decodes that plane of block t - j at step t, receiving
+//      the block's significance count N from lane j - 1 (the plane above)
+//      through a DPP lane shift; plane words go to a per-workgroup scratch;
+//   3. per subband, 4 blocks per wave: a 64x64 bit transpose turns plane
+//      words back into coefficients, inverse coefficient order and
+//      negabinary, inverse zfp lifts (LDS transposes), dequantisation into a
+//      dense LDS subband array, then into the register columns and faces;
+//   4. inverse subband weights and the inverse CDF5/3 on chip (brick32.hpp);
+//   5. the data region goes to the output grid or to the level's dense grid
+//      that the next finer level reads its subband-0 interiors from.
+#pragma once
+
+#include "code32.hpp"
+
+namespace idx2g {
+namespace b32 {
+
+constexpr int kDecStreamBytes = 40 * 1024;
+
+struct DecLds32 {
+  union {
+    double V[17 * 17 * 17];
+  } v;
+  i64 T[kTh / 64][4][64];
+  uint32_t sbuf[kDecStreamBytes / 4 + 8];  // staged streams (+ read padding)
+  u64 soff[8][kMaxKeys];                   // blob offset of each stream
+  uint32_t sstart[8][kMaxKeys];            // byte offset of each stream in sbuf
+  uint32_t ssize[8][kMaxKeys];
+  short E[8][128];
+  unsigned char nbps[8][128];  // planes decoded per block
+  unsigned char nsave[8][128];
+  unsigned char ssave[8][128];
+  int klo[8], khi[8];
+  unsigned char p3[64], p2[16];
+  int total, staged;
+};
+
+// 64 bits of a byte stream at bit pos (words padded by >= 8 bytes)
+__device__ __forceinline__ u64 peek64w(const uint32_t* w32, u64 pos) {
+  const uint32_t* w = w32 + (pos >> 5);
+  const int sh = int(pos & 31);
+  const u64 lo = (u64)w[0] | ((u64)w[1] << 32);
+  if (!sh) return lo;
+  return (lo >> sh) | ((u64)w[2] << (64 - sh));
+}
+
+// One bit plane (Decode, Zfp.h:988-1059) from 32-bit words: plane word.
+__device__ __forceinline__ u64 decode_plane_w(const uint32_t* w32, u64& pos, int& n, int nv) {
+  u64 X = 0;
+  if (n > 0) {
+    const u64 w = peek64w(w32, pos);
+    X = n >= 64 ? w : (w & ((1ull << n) - 1));
+    pos += n;
+  }
+  while (n < nv) {
+    u64 w = peek64w(w32, pos);
+    if (!(w & 1)) { pos += 1; break; }  // group insignificant
+    pos += 1;
+    w >>= 1;
+    const int lim = nv - 1 - n;  // bits examined before the implicit last one
+    const int z = w ? __ffsll((long long)w) - 1 : 64;
+    if (z < lim) { n += z; pos += z + 1; }
+    else { pos += lim; n = nv - 1; }
+    X |= 1ull << n;
+    ++n;
+  }
+  return X;
+}
+
+__device__ __forceinline__ int lane_prev_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+
+// Subband values of V into the register columns and the faces (inverse of
+// stage_subband).
+template <int HZ>
+__device__ __forceinline__ void unstage_cols(double (&c)[kN], const double* V, int i, int j, int dx, int dy, int dz) {
+  const double* q = V + j * dx + i;
+#pragma unroll
+  for (int k = 0; k < 17; ++k)
+    if (k < dz) c[HZ + 2 * k] = q[k * dx * dy];
+}
+
+__device__ __forceinline__ void unstage_subband(double (&c)[kN], Lds& S, const double* V, const Pos& p, int hx, int hy,
+                                                int hz) {
+  const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+  if ((p.x & 1) == hx && (p.y & 1) == hy) {
+    if (hz) unstage_cols<1>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
+    else unstage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
+  }
+  for (int t = p.tid; t < 17 * 16 * 2 + 17; t += kTh) {
+    if (t < 272) {
+      if (hx) continue;
+      const int k = t >> 4, j = t & 15;
+      if (k < dz) S.ex[(hz + 2 * k) * 32 + hy + 2 * j] = V[(k * dy + j) * dx + 16];
+    } else if (t < 544) {
+      if (hy) continue;
+      const int k = (t - 272) >> 4, i = (t - 272) & 15;
+      if (k < dz) S.ey[(hz + 2 * k) * 32 + hx + 2 * i] = V[(k * dy + 16) * dx + i];
+    } else {
+      if (hx || hy) continue;
+      const int k = t - 544;
+      if (k < dz) S.cc[hz + 2 * k] = V[(k * dy + 16) * dx + 16];
+    }
+  }
+}
+
+// Steps 1-2: stream staging and the pipelined walk of all subbands.
+// XS: per-workgroup plane words [sb][block][npl].
+__device__ __forceinline__ void walk_brick(const DecArgs& a, DecLds32& D, const Pos& p, u64 lin, int dense,
+                                           const uint32_t* s_blocks, u64* XS) {
+  const BrickConst& k = a.k;
+  const int npl = k.dtype ? 64 : 32;
+  // exponents and key ranges
+  if (p.tid < 8) { D.klo[p.tid] = 0x7fffffff; D.khi[p.tid] = -0x7fffffff; }
+  __syncthreads();
+  for (int i = p.tid; i < 8 * 128; i += kTh) {
+    const int sb = i >> 7, b = i & 127;
+    D.nbps[sb][b] = 0;
+    D.nsave[sb][b] = 0;
+    D.ssave[sb][b] = 0;
+    if (b >= a.blk_n[sb]) continue;
+    const uint32_t bw = s_blocks[i];
+    if ((bw >> 24) & 1) { D.E[sb][b] = kSkip; continue; }
+    const int ci = int(bw >> 25);
+    const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
+    const int e = k.dtype ? int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023 : int(a.exps[row]) - 127;
+    D.E[sb][b] = (short)e;
+    const int stop = max(64 - k.endbp, 57 + a.etol - e);
+    if (stop <= 63 && ((a.mask >> sb) & 1)) {
+      atomicMax(&D.khi[sb], 63 + e + kBpBiasDev);
+      atomicMin(&D.klo[sb], stop + e + kBpBiasDev);
+    }
+  }
+  __syncthreads();
+  // stream sizes and blob offsets (directory), their layout in LDS
+  __shared__ uint32_t wsum[kTh / 64];
+  {
+    const int sb = p.tid >> 7, kk = p.tid & 127;  // 8 x 128 slots = 1024 threads
+    uint32_t sz = 0;
+    u64 off = 0;
+    if (sb < k.nsb && D.khi[sb] >= D.klo[sb]) {
+      const int key = D.klo[sb] + kk;
+      if (key <= D.khi[sb] && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
+        const u64 d = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
+        if (d) {
+          sz = dir_size(d);
+          off = dir_offset(d);
+        }
+      }
+    }
+    D.ssize[sb][kk] = sz;
+    D.soff[sb][kk] = off;
+    const uint32_t incl = wave_scan_add(sz, p.lane);
+    if (p.lane == 63) wsum[p.w] = incl;
+    __syncthreads();
+    uint32_t base = 0;
+    for (int w = 0; w < p.w; ++w) base += wsum[w];
+    D.sstart[sb][kk] = base + incl - sz;
+    if (p.tid == kTh - 1) {
+      bool wide = false;  // a subband with more keys than the stream table holds
+      for (int s = 0; s < k.nsb; ++s) wide |= D.khi[s] - D.klo[s] + 1 > kMaxKeys;
+      D.total = (int)(base + incl);
+      D.staged = !wide && base + incl <= (uint32_t)kDecStreamBytes;
+    }
+  }
+  __syncthreads();
+  const bool staged = D.staged;
+  if (staged) {
+    uint8_t* dst = reinterpret_cast<uint8_t*>(D.sbuf);
+    for (int j = p.w; j < 8 * 128; j += kTh / 64) {
+      const int sb = j >> 7, kk = j & 127;
+      const uint32_t sz = D.ssize[sb][kk];
+      if (!sz) continue;
+      const uint8_t* src = a.blob + D.soff[sb][kk];
+      uint8_t* o = dst + D.sstart[sb][kk];
+      for (uint32_t b = p.lane; b < sz; b += 64) o[b] = src[b];
+    }
+    if (p.tid < 4) D.sbuf[((D.total + 3) >> 2) + p.tid] = 0;
+  }
+  __syncthreads();
+  // the walk: wave sb walks subband sb
+  const int sb = p.w;
+  if (sb < k.nsb && ((a.mask >> sb) & 1) && D.khi[sb] >= D.klo[sb]) {
+    const int lane = p.lane;
+    const int klo = D.klo[sb], khi = D.khi[sb];
+    const int nblk = a.blk_n[sb];
+    const uint32_t* blocks = s_blocks + sb * 128;
+    u64* xs = XS + (size_t)sb * 128 * npl;
+    for (int g0 = khi; g0 >= klo; g0 -= 64) {
+      const int key = g0 - lane;
+      const bool kact = key >= klo;
+      const int kk = key - klo;
+      u64 dentry = 0;
+      if (staged) {
+        if (kact && kk < kMaxKeys && D.ssize[sb][kk]) dentry = 1;
+      } else if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
+        dentry = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
+      }
+      const bool have = dentry != 0;
+      const uint32_t* w32;
+      u64 pos;
+      if (staged) {
+        w32 = D.sbuf;
+        pos = have ? (u64)D.sstart[sb][kk] * 8 : 0;
+      } else {
+        w32 = reinterpret_cast<const uint32_t*>(a.blob);
+        pos = have ? dir_offset(dentry) * 8 : 0;
+      }
+      int n_out = 0, stop_out = 0;
+      for (int t = 0; t < nblk + 64; ++t) {
+        const int n_left = lane_prev_i(n_out);
+        const int s_left = lane_prev_i(stop_out);
+        const int b = t - lane;
+        const bool inb = b >= 0 && b < nblk;
+        const int e = inb ? D.E[sb][b] : kSkip;
+        if (kact && inb && e != kSkip) {
+          const int n_in = lane == 0 ? (g0 == khi ? 0 : D.nsave[sb][b]) : n_left;
+          const int st_in = lane == 0 ? (g0 == khi ? 0 : D.ssave[sb][b]) : s_left;
+          const int bp = key - kBpBiasDev - e;
+          const int stop = max(64 - k.endbp, 57 + a.etol - e);
+          n_out = n_in;
+          stop_out = st_in;
+          if (bp <= 63 && bp >= stop && !st_in) {
+            if (!have) {
+              stop_out = 1;  // missing chunk: the block stops decoding planes
+            } else {
+              const uint32_t bw = blocks[b];
+              const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
+              int n = n_in;
+              const u64 X = decode_plane_w(w32, pos, n, 1 << (2 * nd));
+              n_out = n;
+              xs[b * npl + (63 - bp)] = X;
+              D.nbps[sb][b] = (unsigned char)(64 - bp);
+            }
+          } else if (bp > 63) {
+            n_out = 0;
+          }
+          if (lane == 63 || key == klo) {
+            D.nsave[sb][b] = (unsigned char)n_out;
+            D.ssave[sb][b] = (unsigned char)stop_out;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Step 3 for subband sb: plane words -> values in D.v.V (zeroed first; the
+// parent's interior already copied for subband 0 below the coarsest level).
+__device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
+                                               const uint32_t* s_blocks, const u64* XS) {
+  const BrickConst& k = a.k;
+  const int npl = k.dtype ? 64 : 32;
+  const int lane = p.lane, q = lane >> 4, r = lane & 15;
+  const int nblk = a.blk_n[sb];
+  const uint32_t* blocks = s_blocks + sb * 128;
+  const u64* xs = XS + (size_t)sb * 128 * npl;
+  i64* T = &D.T[p.w][0][0];
+  for (int g = p.w; g * 4 < nblk; g += kTh / 64) {
+    // this lane's row-mode block
+    const int b = g * 4 + q;
+    const bool bin = b < nblk;
+    const uint32_t bw = bin ? blocks[b] : (1u << 24);
+    const int o3[3] = {int(bw & 31), int((bw >> 5) & 31), int((bw >> 10) & 31)};
+    const int bd[3] = {int((bw >> 15) & 7), int((bw >> 18) & 7), int((bw >> 21) & 7)};
+    const bool ux = bd[0] > 1, uy = bd[1] > 1, uz = bd[2] > 1;
+    const int nd = ux + uy + uz;
+    const int e = bin ? D.E[sb][b] : kSkip;
+    bool live = bin && e != kSkip;
+    if (live) {
+      const int stop = max(64 - k.endbp, 57 + a.etol - e);
+      const int np = stop > 63 ? 0 : 64 - stop;
+      bool bypass = false;
+      for (int d = 0; d < 3; ++d) bypass |= (o3[d] % a.spacing[sb][d]) != 0;
+      live = np > 0 && !bypass;
+    }
+    const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
+                         __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
+    // plane words -> coefficients (bit transpose), inverse order, negabinary
+    if (k.dtype == 0) {
+#pragma unroll
+      for (int pr = 0; pr < 2; ++pr) {
+        const int slot = 2 * pr + (lane >> 5);
+        const int blk = g * 4 + slot;
+        const int pp = 31 - (lane & 31);
+        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[blk * 32 + pp] : 0ull;
+        const u64 R = bit_transpose(X, lane);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int sl = 2 * pr + h, ndq = nd_s[sl];
+          const u64 u = h == 0 ? (R << 32) : (R & 0xFFFFFFFF00000000ull);
+          const int c = lane;
+          if (c < (1 << (2 * ndq))) {
+            const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
+            T[sl * 64 + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const int blk = g * 4 + qq;
+        const int pp = 63 - lane;
+        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[blk * 64 + pp] : 0ull;
+        const u64 u = bit_transpose(X, lane);
+        const int ndq = nd_s[qq], c = lane;
+        if (c < (1 << (2 * ndq))) {
+          const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
+          T[qq * 64 + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+        }
+      }
+    }
+    wave_sync_lds();
+    i64* Tq = T + q * 64;
+    {  // third axis (inverse lifts run z, y, x)
+      const int j0 = r & 3, j1 = r >> 2;
+      i64 y4[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * j1 + 16 * i];
+      if (nd == 3) {
+        zfp_ilift4(y4[0], y4[1], y4[2], y4[3]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * j1 + 16 * i] = y4[i];
+      }
+    }
+    wave_sync_lds();
+    {  // second axis
+      const int j0 = r & 3, j2 = r >> 2;
+      i64 y4[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + 16 * j2];
+      if (nd >= 2) {
+        zfp_ilift4(y4[0], y4[1], y4[2], y4[3]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + 16 * j2] = y4[i];
+      }
+    }
+    wave_sync_lds();
+    // row mode: first axis, dequantise, into V (Dequantize, Function.h:139-149)
+    {
+      const int dxy = dx * dy;
+      const int s0 = ux ? 1 : (uy ? dx : dxy);
+      const int s1 = ux ? (uy ? dx : dxy) : (uy ? dxy : 0);
+      const int s2 = (ux && uy && uz) ? dxy : 0;
+      const int i1 = r & 3, i2 = r >> 2;
+      const bool rowok = live && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
+      const int n0 = nd >= 1 ? 4 : 1;
+      i64 w4[4];
+#pragma unroll
+      for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[(i1 + 4 * i2) * 4 + i0];
+      if (nd >= 1) zfp_ilift4(w4[0], w4[1], w4[2], w4[3]);
+      const double scale = 1.0 / pow2d((63 - nd) - 1 - e);
+      const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
+#pragma unroll
+      for (int i0 = 0; i0 < 4; ++i0)
+        if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = scale * i64_to_f64(w4[i0]);
+    }
+    wave_sync_lds();
+  }
+}
+
+}  // namespace b32
+}  // namespace idx2g

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
+#include "dec32.hpp"
 
 namespace idx2g {
 
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
   const int prec3[4] = {63, 62, 61, 60};
 
   for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
-    double* F = a.bricks + (size_t)bi * k.nvox;
+    double* F = a.scratch + (size_t)blockIdx.x * k.nvox;
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
     const u64 lin = a.lin[bi];
     const int dense = a.dense[bi];
@@ -98,14 +99,13 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
       const DGrid& g = k.sb[sb];
       // parent's decoded interior into subband 0 (CopyExtentGrid, Volume.h:1153-1173)
       if (sb == 0 && !a.coarsest) {
-        const double* P = a.parents + (size_t)a.parent_slot[bi] * k.nvox;
-        int lp[3];
-        for (int d = 0; d < 3; ++d) lp[d] = a.brick[d] > 1 ? (b3[d] & 1) : 0;
+        // the parent's decoded data at this brick's place in the level+1 grid
         const int n = a.sbne[0] * a.sbne[1] * a.sbne[2];
         for (int i = tid; i < n; i += blockDim.x) {
           int x = i % a.sbne[0], y = (i / a.sbne[0]) % a.sbne[1], z = i / (a.sbne[0] * a.sbne[1]);
-          F[vox(k, g.f[0] + g.s[0] * x, g.f[1] + g.s[1] * y, g.f[2] + g.s[2] * z)] =
-              P[vox(k, lp[0] * a.sbne[0] + x, lp[1] * a.sbne[1] + y, lp[2] * a.sbne[2] + z)];
+          const size_t src = ((size_t)(b3[2] * a.sbne[2] + z) * a.pgdims[1] + (b3[1] * a.sbne[1] + y)) * a.pgdims[0] +
+                             (b3[0] * a.sbne[0] + x);
+          F[vox(k, g.f[0] + g.s[0] * x, g.f[1] + g.s[1] * y, g.f[2] + g.s[2] * z)] = a.pgrid[src];
         }
       }
       const int nblk = a.blk_n[sb];
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
           bool have = false;
           if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
             const u64 d = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
-            if (d) { pos = (d - 1) * 8; have = true; }
+            if (d) { pos = dir_offset(d) * 8; have = true; }
           }
           int n_out = 0;
           bool stop_out = false;
@@ -215,6 +215,17 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
       __syncthreads();
     }
     inverse_brick(k, F, a.winv, a.coarsest);
+    // data region into this level's grid (read by the next finer level)
+    if (a.cgrid) {
+      int ext[3];
+      for (int d = 0; d < 3; ++d) ext[d] = max(0, min(a.brick[d], a.cgdims[d] - b3[d] * a.brick[d]));
+      const int n = ext[0] * ext[1] * ext[2];
+      for (int i = tid; i < n; i += blockDim.x) {
+        const int x = i % ext[0], y = (i / ext[0]) % ext[1], z = i / (ext[0] * ext[1]);
+        a.cgrid[((size_t)(b3[2] * a.brick[2] + z) * a.cgdims[1] + (b3[1] * a.brick[1] + y)) * a.cgdims[0] +
+                (b3[0] * a.brick[0] + x)] = F[vox(k, x, y, z)];
+      }
+    }
     // ---- copy into the output grid: Crop(OutGrid, BrickGrid) ----
     if (a.out) {
       int cf[3], cd[3], bf[3];
@@ -247,6 +258,93 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
           else ((float*)a.out)[oi] = (float)v;
         }
       }
+    }
+    __syncthreads();
+  }
+}
+
+// Full 32^3 bricks of a 3D dataset (dec32.hpp): one brick per iteration,
+// 1024 threads, the brick on chip.
+__global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restrict__ ap) {
+  const DecArgs& a = *ap;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ uint32_t s_blocks[8 * 128];
+  __shared__ double w8[8];
+  b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
+  b32::DecLds32& D = *reinterpret_cast<b32::DecLds32*>(&S.u);
+  const BrickConst& k = a.k;
+  for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
+    const int s = i >> 7, b = i & 127;
+    s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
+  }
+  if (threadIdx.x < 8) {
+    const int s = threadIdx.x;
+    const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
+    w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.winv[s];
+  }
+  u64* XS = reinterpret_cast<u64*>(a.scratch) + (size_t)blockIdx.x * (8 * 128 * 64);
+  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+    const b32::Pos p = b32::pos();
+    const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
+    const u64 lin = a.lin[bi];
+    const int dense = a.dense[bi];
+    if (p.tid < 64) D.p3[p.tid] = a.perm.p3[p.tid];
+    if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
+    b32::walk_brick(a, D, p, lin, dense, s_blocks, XS);
+    double c[b32::kN];
+    for (int sb = 0; sb < 8; ++sb) {
+      const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
+      const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+      for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
+      __syncthreads();
+      if ((a.mask >> sb) & 1) {
+        if (sb == 0 && !a.coarsest) {
+          // parent's decoded data: subband-0 interior (CopyExtentGrid, Volume.h:1153-1173)
+          for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
+            const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+            D.v.V[(z * 17 + y) * 17 + x] =
+                a.pgrid[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
+          }
+          __syncthreads();
+        }
+        b32::inverse_blocks(a, sb, dx, dy, D, p, s_blocks, XS);
+      }
+      __syncthreads();
+      b32::unstage_subband(c, S, D.v.V, p, hx, hy, hz);
+      __syncthreads();
+    }
+    b32::weigh(c, S, p, w8);
+    b32::inverse(c, S, p);
+    if (a.out) {
+      // Crop(OutGrid, BrickGrid) (idx2Decode.cpp:431-445), one element per (thread, z)
+      int o[2];
+      bool ok2 = true;
+      const int xy[2] = {p.x, p.y};
+      for (int d = 0; d < 2; ++d) {
+        const int gp = b3[d] * a.B3[d] + xy[d] * a.bstride;
+        const int olast = a.og_f[d] + (a.og_d[d] - 1) * a.og_s[d];
+        ok2 = ok2 && gp >= a.og_f[d] && gp <= olast && (gp - a.og_f[d]) % a.og_s[d] == 0;
+        o[d] = (gp - a.og_f[d]) / a.og_s[d];
+      }
+      if (ok2) {
+        const int olast = a.og_f[2] + (a.og_d[2] - 1) * a.og_s[2];
+        const size_t row = (size_t)o[1] * a.og_d[0] + o[0];
+        const size_t plane = (size_t)a.og_d[1] * a.og_d[0];
+#pragma unroll
+        for (int z = 0; z < 32; ++z) {
+          const int gp = b3[2] * a.B3[2] + z * a.bstride;
+          if (gp >= a.og_f[2] && gp <= olast && (gp - a.og_f[2]) % a.og_s[2] == 0) {
+            const size_t oi = (size_t)((gp - a.og_f[2]) / a.og_s[2]) * plane + row;
+            if (k.dtype) ((double*)a.out)[oi] = c[z];
+            else ((float*)a.out)[oi] = (float)c[z];
+          }
+        }
+      }
+    } else {
+      const size_t plane = (size_t)a.cgdims[1] * a.cgdims[0];
+      double* dst = a.cgrid + ((size_t)(b3[2] * 32) * a.cgdims[1] + (b3[1] * 32 + p.y)) * a.cgdims[0] + (b3[0] * 32 + p.x);
+#pragma unroll
+      for (int z = 0; z < 32; ++z) dst[z * plane] = c[z];
     }
     __syncthreads();
   }

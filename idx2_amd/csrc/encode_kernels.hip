@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(256) k_chunk_write(const uint32_t* idx, const 
       int i = 0;
       do { uint8_t byte = v & 0x7f; v >>= 7; vp[i++] = byte | (v ? 0x80 : 0); } while (v);
       const u64 dst = c.off + payload_start + doff;
-      drow[b] = dst + 1;
+      drow[b] = dir_entry(dst, sz);
       const int slot = atomicAdd(&s_n, 1);
       s_src[slot] = src;
       s_dst[slot] = dst;

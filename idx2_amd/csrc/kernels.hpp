@@ -17,6 +17,12 @@ __host__ __device__ inline uint32_t pack_block(const int org[3], const int dims[
                   (dims[2] << 21) | (skip << 24)) | (uint32_t(ci) << 25);
 }
 
+// Chunk directory entry of one (subband, key, brick) stream in a level's
+// chunk blob: (byte offset + 1) | size << 40; 0 = no stream.
+__host__ __device__ inline u64 dir_entry(u64 off, u64 size) { return (off + 1) | (size << 40); }
+__host__ __device__ inline u64 dir_offset(u64 e) { return (e & ((1ull << 40) - 1)) - 1; }
+__host__ __device__ inline uint32_t dir_size(u64 e) { return (uint32_t)(e >> 40); }
+
 struct SbEntry {  // one (brick, subband, key) stream in the arena
   u64 off;
   uint32_t size;
@@ -82,9 +88,11 @@ struct DecArgs {
   const u64* lin;        // [n] linear ids
   const int* dense;      // [n] dense ids (exponent rows)
   int nb;
-  double* bricks;        // [n][nvox] output bricks of this batch
-  const double* parents; // decoded bricks of level+1
-  const int* parent_slot;// [n] slot in parents
+  double* scratch;       // per workgroup: a work volume (generic kernel) or plane words (b32)
+  const double* pgrid;   // level+1 decoded data (dense f64; subband-0 interiors of this level)
+  int pgdims[3];
+  double* cgrid;         // this level's decoded data when it is not the output level
+  int cgdims[3];
   int sbne[3];
   const uint32_t* blocks;
   int blk_off[8], blk_n[8], ncoded[8];
