@@ -5,7 +5,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <unordered_map>
 
 #include "host.hpp"
 
@@ -77,6 +79,63 @@ PermTables make_perm_tables() {
   return t;
 }
 
+namespace {
+struct DevCache {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void*>> free_;      // device -> size -> block
+  std::unordered_map<void*, std::pair<int, size_t>> live;  // block -> (device, size)
+};
+DevCache& dev_cache() {
+  static DevCache* c = new DevCache();  // never destroyed: blocks live until the process ends
+  return *c;
+}
+size_t round_bytes(size_t n) {
+  if (n < (1u << 20)) return (n + 255) & ~size_t(255);
+  return (n + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+}
+}  // namespace
+
+hipError_t dev_alloc(void** p, size_t bytes) {
+  DevCache& c = dev_cache();
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const size_t n = round_bytes(bytes ? bytes : 1);
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto& fl = c.free_[dev];
+    auto it = fl.lower_bound(n);
+    if (it != fl.end() && it->first <= n + n / 2 + (2u << 20)) {
+      *p = it->second;
+      c.live[*p] = {dev, it->first};
+      fl.erase(it);
+      return hipSuccess;
+    }
+  }
+  e = hipMalloc(p, n);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> g(c.mu);  // give the cached blocks back and retry
+    for (auto& kv : c.free_[dev]) (void)hipFree(kv.second);
+    c.free_[dev].clear();
+    e = hipMalloc(p, n);
+  }
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(c.mu);
+  c.live[*p] = {dev, n};
+  return hipSuccess;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  auto it = c.live.find(p);
+  if (it == c.live.end()) { (void)hipFree(p); return; }
+  c.free_[it->second.first].emplace(it->second.second, p);
+  c.live.erase(it);
+}
+
 int upload_block_tables(idx2g_dataset* ds) {
   const Layout& L = ds->L;
   std::vector<uint32_t> all;
@@ -89,7 +148,7 @@ int upload_block_tables(idx2g_dataset* ds) {
         if (!b.skip) ++ci;
       }
     }
-  HIPCHK(hipMalloc(&ds->d_blocks, all.size() * 4 + 4));
+  HIPCHK(dev_alloc((void**)&ds->d_blocks, all.size() * 4 + 4));
   HIPCHK(hipMemcpy(ds->d_blocks, all.data(), all.size() * 4, hipMemcpyHostToDevice));
   return IDX2G_OK;
 }

@@ -51,6 +51,48 @@ __device__ __forceinline__ u64 peek64w(const uint32_t* w32, u64 pos) {
 }
 
 // One bit plane (Decode, Zfp.h:988-1059) from 32-bit words: plane word.
+// A 64-bit window is re-read only when the next group test may need bits
+// beyond it.
+__device__ __forceinline__ u64 decode_plane_win(const uint32_t* w32, u64& pos, int& n, int nv) {
+  u64 X = 0;
+  u64 w = peek64w(w32, pos);
+  int avail = 64;
+  if (n > 0) {
+    if (n >= 64) {
+      X = w;
+      pos += 64;
+      avail = 0;
+    } else {
+      X = w & ((1ull << n) - 1);
+      pos += n;
+      w >>= n;
+      avail -= n;
+    }
+  }
+  while (n < nv) {
+    const int lim = nv - 1 - n;  // bits examined before the implicit last one
+    if (avail < lim + 1) {
+      w = peek64w(w32, pos);
+      avail = 64;
+    }
+    const bool sig = w & 1;
+    pos += 1;
+    w >>= 1;
+    avail -= 1;
+    if (!sig) break;  // group insignificant
+    const int z = w ? __ffsll((long long)w) - 1 : 64;
+    int adv;
+    if (z < lim) { n += z; adv = z + 1; }
+    else { n = nv - 1; adv = lim; }
+    pos += adv;
+    w = adv >= 64 ? 0 : w >> adv;
+    avail -= adv;
+    X |= 1ull << n;
+    ++n;
+  }
+  return X;
+}
+
 __device__ __forceinline__ u64 decode_plane_w(const uint32_t* w32, u64& pos, int& n, int nv) {
   u64 X = 0;
   if (n > 0) {
@@ -155,12 +197,15 @@ __device__ __forceinline__ void walk_brick(const DecArgs& a, DecLds32& D, const 
     }
     D.ssize[sb][kk] = sz;
     D.soff[sb][kk] = off;
-    const uint32_t incl = wave_scan_add(sz, p.lane);
+    // each stream gets a word-aligned region (same byte phase as in the blob)
+    const uint32_t reg = sz ? ((uint32_t)(off & 3) + sz + 3) & ~3u : 0u;
+    const uint32_t incl = wave_scan_add(reg, p.lane);
     if (p.lane == 63) wsum[p.w] = incl;
     __syncthreads();
     uint32_t base = 0;
     for (int w = 0; w < p.w; ++w) base += wsum[w];
-    D.sstart[sb][kk] = base + incl - sz;
+    const uint32_t rstart = base + incl - reg;
+    D.sstart[sb][kk] = rstart + (uint32_t)(off & 3);
     if (p.tid == kTh - 1) {
       bool wide = false;  // a subband with more keys than the stream table holds
       for (int s = 0; s < k.nsb; ++s) wide |= D.khi[s] - D.klo[s] + 1 > kMaxKeys;
@@ -169,20 +214,29 @@ __device__ __forceinline__ void walk_brick(const DecArgs& a, DecLds32& D, const 
     }
   }
   __syncthreads();
-  const bool staged = D.staged;
-  if (staged) {
-    uint8_t* dst = reinterpret_cast<uint8_t*>(D.sbuf);
-    for (int j = p.w; j < 8 * 128; j += kTh / 64) {
-      const int sb = j >> 7, kk = j & 127;
+    const bool staged = D.staged;
+    if (staged) {
+      // one thread per stream, aligned 32-bit words (all loads issued first)
+      const int sb = p.tid >> 7, kk = p.tid & 127;
       const uint32_t sz = D.ssize[sb][kk];
-      if (!sz) continue;
-      const uint8_t* src = a.blob + D.soff[sb][kk];
-      uint8_t* o = dst + D.sstart[sb][kk];
-      for (uint32_t b = p.lane; b < sz; b += 64) o[b] = src[b];
+      if (sz) {
+        const u64 off = D.soff[sb][kk];
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.blob) + (off >> 2);
+        uint32_t* dst = D.sbuf + (D.sstart[sb][kk] >> 2);
+        const uint32_t nw = ((uint32_t)(off & 3) + sz + 3) >> 2;
+        uint32_t i = 0;
+        for (; i + 8 <= nw; i += 8) {
+          uint32_t t8[8];
+  #pragma unroll
+          for (int j = 0; j < 8; ++j) t8[j] = src[i + j];
+  #pragma unroll
+          for (int j = 0; j < 8; ++j) dst[i + j] = t8[j];
+        }
+        for (; i < nw; ++i) dst[i] = src[i];
+      }
+      if (p.tid < 4) D.sbuf[((uint32_t)D.total >> 2) + p.tid] = 0;
     }
-    if (p.tid < 4) D.sbuf[((D.total + 3) >> 2) + p.tid] = 0;
-  }
-  __syncthreads();
+    __syncthreads();
   // the walk: wave sb walks subband sb
   const int sb = p.w;
   if (sb < k.nsb && ((a.mask >> sb) & 1) && D.khi[sb] >= D.klo[sb]) {
@@ -212,12 +266,22 @@ __device__ __forceinline__ void walk_brick(const DecArgs& a, DecLds32& D, const 
         pos = have ? dir_offset(dentry) * 8 : 0;
       }
       int n_out = 0, stop_out = 0;
-      for (int t = 0; t < nblk + 64; ++t) {
+      const int nsteps = nblk + min(64, g0 - klo + 1);
+      int e_next = (-lane >= 0 && -lane < nblk) ? D.E[sb][0] : kSkip;
+      uint32_t bw_next = blocks[0];
+      for (int t = 0; t < nsteps; ++t) {
         const int n_left = lane_prev_i(n_out);
         const int s_left = lane_prev_i(stop_out);
         const int b = t - lane;
         const bool inb = b >= 0 && b < nblk;
-        const int e = inb ? D.E[sb][b] : kSkip;
+        const int e = e_next;
+        const uint32_t bwc = bw_next;
+        {  // prefetch the next step's block exponent and table word
+          const int bn = b + 1;
+          const bool inn = bn >= 0 && bn < nblk;
+          e_next = inn ? D.E[sb][bn] : kSkip;
+          bw_next = blocks[inn ? bn : 0];
+        }
         if (kact && inb && e != kSkip) {
           const int n_in = lane == 0 ? (g0 == khi ? 0 : D.nsave[sb][b]) : n_left;
           const int st_in = lane == 0 ? (g0 == khi ? 0 : D.ssave[sb][b]) : s_left;
@@ -229,10 +293,9 @@ __device__ __forceinline__ void walk_brick(const DecArgs& a, DecLds32& D, const 
             if (!have) {
               stop_out = 1;  // missing chunk: the block stops decoding planes
             } else {
-              const uint32_t bw = blocks[b];
-              const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
+              const int nd = (((bwc >> 15) & 7) > 1) + (((bwc >> 18) & 7) > 1) + (((bwc >> 21) & 7) > 1);
               int n = n_in;
-              const u64 X = decode_plane_w(w32, pos, n, 1 << (2 * nd));
+              const u64 X = decode_plane_win(w32, pos, n, 1 << (2 * nd));
               n_out = n;
               xs[b * npl + (63 - bp)] = X;
               D.nbps[sb][b] = (unsigned char)(64 - bp);

@@ -498,6 +498,30 @@ __global__ void __launch_bounds__(256) k_chunk_size(const uint32_t* idx, const S
   }
 }
 
+// Byte copy by one thread: aligned 32-bit destination words, each assembled
+// from two aligned source words (sources are padded by >= 4 bytes).
+__device__ inline void copy_bytes(uint8_t* d, const uint8_t* s, uint32_t n) {
+  uint32_t i = 0;
+  const uint32_t head = min(n, (4u - (uint32_t)((uintptr_t)d & 3)) & 3u);
+  for (; i < head; ++i) d[i] = s[i];
+  const uint32_t nw = (n - i) >> 2;
+  uint32_t* dw = reinterpret_cast<uint32_t*>(d + i);
+  const uint8_t* sp = s + i;
+  const uint32_t sh = (uint32_t)((uintptr_t)sp & 3);
+  const uint32_t* sw = reinterpret_cast<const uint32_t*>(sp - sh);
+  uint32_t k = 0;
+  for (; k + 8 <= nw; k += 8) {
+    uint32_t t[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[j] = sw[k + j];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dw[k + j] = __builtin_amdgcn_alignbyte(t[j + 1], t[j], sh);
+  }
+  for (; k < nw; ++k) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
+  i += nw * 4;
+  for (; i < n; ++i) d[i] = s[i];
+}
+
 // One workgroup per non-empty chunk (in blob order): write the header, the
 // varbyte sizes and the payloads, and record where each brick's stream
 // starts (decode directory: blob offset + 1).
@@ -538,9 +562,7 @@ __global__ void __launch_bounds__(256) k_chunk_write(const uint32_t* idx, const 
   // total varbyte bytes of the sizes section
   __shared__ unsigned long long s_vbtotal, s_vboff, s_doff;
   __shared__ uint32_t s1[256], s2[256];
-  __shared__ u64 s_src[256], s_dst[256];
-  __shared__ uint32_t s_len[256];
-  __shared__ int s_n;
+
   if (tid == 0) { s_vbtotal = 0; s_vboff = 0; s_doff = 0; }
   __syncthreads();
   unsigned long long vbl = 0;
@@ -574,8 +596,6 @@ __global__ void __launch_bounds__(256) k_chunk_write(const uint32_t* idx, const 
     }
     const u64 vboff = s_vboff + s1[tid] - v1;
     const u64 doff = s_doff + s2[tid] - sz;
-    if (tid == 0) s_n = 0;
-    __syncthreads();
     if (sz) {
       uint8_t* vp = out + sizes_off + vboff;
       u64 v = sz;
@@ -583,19 +603,9 @@ __global__ void __launch_bounds__(256) k_chunk_write(const uint32_t* idx, const 
       do { uint8_t byte = v & 0x7f; v >>= 7; vp[i++] = byte | (v ? 0x80 : 0); } while (v);
       const u64 dst = c.off + payload_start + doff;
       drow[b] = dir_entry(dst, sz);
-      const int slot = atomicAdd(&s_n, 1);
-      s_src[slot] = src;
-      s_dst[slot] = dst;
-      s_len[slot] = sz;
-    }
-    __syncthreads();
-    // payload copy: one wave per stream, lanes over bytes
-    for (int i = wave; i < s_n; i += blockDim.x >> 6) {
-      const uint8_t* sp = arena + s_src[i];
-      uint8_t* dp = blob + s_dst[i];
-      for (uint32_t j = lane; j < s_len[i]; j += 64) dp[j] = sp[j];
-    }
-    __syncthreads();
+        copy_bytes(blob + dst, arena + src, sz);  // one thread per stream
+      }
+      __syncthreads();
     if (tid == 255) { s_vboff += s1[255]; s_doff += s2[255]; }
     __syncthreads();
   }

@@ -62,6 +62,13 @@ struct idx2g_dataset {
 namespace idx2g {
 void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, int line);
+
+// Device memory through a per-device cache of freed blocks: repeated
+// encode/decode calls reuse their large buffers instead of paying
+// hipMalloc/hipFree each time.  Blocks are only handed back after the
+// library call that freed them has synchronised its stream.
+hipError_t dev_alloc(void** p, size_t bytes);
+void dev_free(void* p);
 #define HIPCHK(x)                                                  \
   do {                                                             \
     hipError_t e_ = (x);                                           \

@@ -27,7 +27,7 @@ template <class T>
 static int dalloc(T** p, size_t n) {
   *p = nullptr;
   if (n == 0) n = 1;
-  HIPCHK(hipMalloc((void**)p, n * sizeof(T)));
+  HIPCHK(dev_alloc((void**)p, n * sizeof(T)));
   return IDX2G_OK;
 }
 // Copy a kernel argument block into the next slot of the context's ring and
@@ -55,10 +55,10 @@ static int stage_args(idx2g_ctx* ctx, const T& a, const T** out) {
   } while (0)
 
 static void free_level(LevelData& lv) {
-  if (lv.d_blob) (void)hipFree(lv.d_blob);
-  if (lv.d_dir) (void)hipFree(lv.d_dir);
-  if (lv.d_exps) (void)hipFree(lv.d_exps);
-  if (lv.d_xyz) (void)hipFree(lv.d_xyz);
+  if (lv.d_blob) dev_free(lv.d_blob);
+  if (lv.d_dir) dev_free(lv.d_dir);
+  if (lv.d_exps) dev_free(lv.d_exps);
+  if (lv.d_xyz) dev_free(lv.d_xyz);
   lv = LevelData();
 }
 
@@ -104,11 +104,11 @@ struct EncodeBuffers {
   unsigned long long arena_cap = 0, entry_cap = 0;
   SbHdr* hdrs[kMaxLevels] = {nullptr};
   void release() {
-    if (arena) (void)hipFree(arena);
-    if (entries) (void)hipFree(entries);
-    if (counters) (void)hipFree(counters);
+    if (arena) dev_free(arena);
+    if (entries) dev_free(entries);
+    if (counters) dev_free(counters);
     for (auto& h : hdrs)
-      if (h) { (void)hipFree(h); h = nullptr; }
+      if (h) { dev_free(h); h = nullptr; }
     arena = nullptr; entries = nullptr; counters = nullptr;
   }
 };
@@ -238,15 +238,15 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
       HIPCHK(hipMemcpyAsync(l0_bytes, B.counters + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     }
     HIPCHK(hipStreamSynchronize(st));
-    (void)hipFree(d_lists);
-    if (level_grid) (void)hipFree(level_grid);
+    dev_free(d_lists);
+    if (level_grid) dev_free(level_grid);
     level_grid = parents;
     level_in = parents;
     for (int d = 0; d < 3; ++d) gd[d] = pd[d];
   }
   HIPCHK(hipStreamSynchronize(st));
-  if (level_grid) (void)hipFree(level_grid);
-  (void)hipFree(scratch);
+  if (level_grid) dev_free(level_grid);
+  dev_free(scratch);
   HIPCHK(hipMemcpy(flags, B.counters, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   return rc;
 }
@@ -266,7 +266,7 @@ static int assemble_level(idx2g_ctx* ctx, idx2g_dataset* ds, int l, EncodeBuffer
   int kr[2];
   HIPCHK(hipMemcpyAsync(kr, d_kr, sizeof kr, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  (void)hipFree(d_kr);
+  dev_free(d_kr);
   if (kr[1] < kr[0]) {  // nothing coded on this level
     lv.klo = 0; lv.nk = 0;
     TRY(dalloc(&lv.d_blob, kBlobPad));
@@ -319,10 +319,10 @@ static int assemble_level(idx2g_ctx* ctx, idx2g_dataset* ds, int l, EncodeBuffer
   HIPCHK(hipStreamSynchronize(st));
   lv.chunks.clear();
   for (int i : order) lv.chunks.push_back(recs[i]);
-  (void)hipFree(d_idx);
-  (void)hipFree(d_lin);
-  (void)hipFree(d_recs);
-  (void)hipFree(d_order);
+  dev_free(d_idx);
+  dev_free(d_lin);
+  dev_free(d_recs);
+  dev_free(d_order);
   return IDX2G_OK;
 }
 
@@ -359,11 +359,11 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
   const void* d_in = input;
   void* d_tmp = nullptr;
   if (!input_is_device) {
-    HIPCHK(hipMalloc(&d_tmp, raw));
+    HIPCHK(dev_alloc(&d_tmp, raw));
     HIPCHK(hipMemcpy(d_tmp, input, raw, hipMemcpyHostToDevice));
     d_in = d_tmp;
   }
-  HIPCHK(hipMalloc(&ds->d_minmax, (size_t)ds->lv[0].nb * 2 * sizeof(double)));
+  HIPCHK(dev_alloc((void**)&ds->d_minmax, (size_t)ds->lv[0].nb * 2 * sizeof(double)));
   ds->have_range = true;
   int nbt = 0;
   for (int l = 0; l < L.nlevels; ++l) nbt += ds->lv[l].nb;
@@ -407,7 +407,7 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
     break;
   }
   B.release();
-  if (d_tmp) (void)hipFree(d_tmp);
+  if (d_tmp) dev_free(d_tmp);
   if (rc) {
     for (int l = 0; l < L.nlevels; ++l) free_level(ds->lv[l]);
     delete ds;
