@@ -21,24 +21,15 @@
 namespace idx2g {
 namespace b32 {
 
-constexpr int kDecStreamBytes = 40 * 1024;
 
 struct DecLds32 {
   union {
     double V[17 * 17 * 17];
   } v;
   i64 T[kTh / 64][4][64];
-  uint32_t sbuf[kDecStreamBytes / 4 + 8];  // staged streams (+ read padding)
-  u64 soff[8][kMaxKeys];                   // blob offset of each stream
-  uint32_t sstart[8][kMaxKeys];            // byte offset of each stream in sbuf
-  uint32_t ssize[8][kMaxKeys];
   short E[8][128];
   unsigned char nbps[8][128];  // planes decoded per block
-  unsigned char nsave[8][128];
-  unsigned char ssave[8][128];
-  int klo[8], khi[8];
   unsigned char p3[64], p2[16];
-  int total, staged;
 };
 
 // 64 bits of a byte stream at bit pos (words padded by >= 8 bytes)
@@ -151,173 +142,201 @@ __device__ __forceinline__ void unstage_subband(double (&c)[kN], Lds& S, const d
   }
 }
 
-// Steps 1-2: stream staging and the pipelined walk of all subbands.
-// XS: per-workgroup plane words [sb][block][npl].
-__device__ __forceinline__ void walk_brick(const DecArgs& a, DecLds32& D, const Pos& p, u64 lin, int dense,
-                                           const uint32_t* s_blocks, u64* XS) {
+// Steps 1-2, one 256-thread workgroup per brick (k_walk_b32): stream
+// staging and the pipelined walks.  Wave w walks subbands 2w and 2w + 1, one
+// per 32-lane segment: lane l of a segment owns key g0 - l of the current
+// group of 32 keys.
+constexpr int kWalkThreads = 256;
+constexpr int kWalkSlots = 64;                // staged streams per subband
+constexpr int kWalkStreamBytes = 20 * 1024;   // LDS stream buffer per brick
+
+struct WalkLds {
+  uint32_t sbuf[kWalkStreamBytes / 4 + 8];
+  u64 soff[8][kWalkSlots];
+  uint32_t sstart[8][kWalkSlots];
+  uint32_t ssize[8][kWalkSlots];
+  short E[8][128];
+  unsigned char nbps[8][128];
+  unsigned char nsave[8][128];
+  unsigned char ssave[8][128];
+  uint32_t wsum[kWalkThreads / 64];
+  int klo[8], khi[8];
+  int total, staged;
+};
+
+// Exponent of block b of subband sb (EncodeBrickSubbandExponents layout),
+// kSkip for blocks that are not coded.
+__device__ __forceinline__ int block_exponent(const DecArgs& a, const uint32_t* s_blocks, int sb, int b, int dense) {
+  const uint32_t bw = s_blocks[sb * 128 + b];
+  if ((bw >> 24) & 1) return kSkip;
+  const int ci = int(bw >> 25);
+  const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (a.k.dtype ? 2 : 1);
+  return a.k.dtype ? int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023 : int(a.exps[row]) - 127;
+}
+
+__device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, const uint32_t* s_blocks) {
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
-  // exponents and key ranges
-  if (p.tid < 8) { D.klo[p.tid] = 0x7fffffff; D.khi[p.tid] = -0x7fffffff; }
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const u64 lin = a.lin[bi];
+  const int dense = a.dense[bi];
+  if (tid < 8) { W.klo[tid] = 0x7fffffff; W.khi[tid] = -0x7fffffff; }
   __syncthreads();
-  for (int i = p.tid; i < 8 * 128; i += kTh) {
+  for (int i = tid; i < 8 * 128; i += kWalkThreads) {
     const int sb = i >> 7, b = i & 127;
-    D.nbps[sb][b] = 0;
-    D.nsave[sb][b] = 0;
-    D.ssave[sb][b] = 0;
-    if (b >= a.blk_n[sb]) continue;
-    const uint32_t bw = s_blocks[i];
-    if ((bw >> 24) & 1) { D.E[sb][b] = kSkip; continue; }
-    const int ci = int(bw >> 25);
-    const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
-    const int e = k.dtype ? int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023 : int(a.exps[row]) - 127;
-    D.E[sb][b] = (short)e;
+    W.nbps[sb][b] = 0;
+    W.nsave[sb][b] = 0;
+    W.ssave[sb][b] = 0;
+    if (sb >= k.nsb || b >= a.blk_n[sb]) continue;
+    const int e = block_exponent(a, s_blocks, sb, b, dense);
+    W.E[sb][b] = (short)e;
+    if (e == kSkip) continue;
     const int stop = max(64 - k.endbp, 57 + a.etol - e);
     if (stop <= 63 && ((a.mask >> sb) & 1)) {
-      atomicMax(&D.khi[sb], 63 + e + kBpBiasDev);
-      atomicMin(&D.klo[sb], stop + e + kBpBiasDev);
+      atomicMax(&W.khi[sb], 63 + e + kBpBiasDev);
+      atomicMin(&W.klo[sb], stop + e + kBpBiasDev);
     }
   }
   __syncthreads();
-  // stream sizes and blob offsets (directory), their layout in LDS
-  __shared__ uint32_t wsum[kTh / 64];
+  // directory entries of the first kWalkSlots keys of each subband, regions in LDS
   {
-    const int sb = p.tid >> 7, kk = p.tid & 127;  // 8 x 128 slots = 1024 threads
-    uint32_t sz = 0;
-    u64 off = 0;
-    if (sb < k.nsb && D.khi[sb] >= D.klo[sb]) {
-      const int key = D.klo[sb] + kk;
-      if (key <= D.khi[sb] && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
-        const u64 d = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
-        if (d) {
-          sz = dir_size(d);
-          off = dir_offset(d);
-        }
-      }
-    }
-    D.ssize[sb][kk] = sz;
-    D.soff[sb][kk] = off;
-    // each stream gets a word-aligned region (same byte phase as in the blob)
-    const uint32_t reg = sz ? ((uint32_t)(off & 3) + sz + 3) & ~3u : 0u;
-    const uint32_t incl = wave_scan_add(reg, p.lane);
-    if (p.lane == 63) wsum[p.w] = incl;
-    __syncthreads();
     uint32_t base = 0;
-    for (int w = 0; w < p.w; ++w) base += wsum[w];
-    const uint32_t rstart = base + incl - reg;
-    D.sstart[sb][kk] = rstart + (uint32_t)(off & 3);
-    if (p.tid == kTh - 1) {
-      bool wide = false;  // a subband with more keys than the stream table holds
-      for (int s = 0; s < k.nsb; ++s) wide |= D.khi[s] - D.klo[s] + 1 > kMaxKeys;
-      D.total = (int)(base + incl);
-      D.staged = !wide && base + incl <= (uint32_t)kDecStreamBytes;
+    for (int i0 = 0; i0 < 8 * kWalkSlots; i0 += kWalkThreads) {
+      const int i = i0 + tid, sb = i / kWalkSlots, kk = i % kWalkSlots;
+      uint32_t sz = 0;
+      u64 off = 0;
+      if (sb < k.nsb && W.khi[sb] >= W.klo[sb]) {
+        const int key = W.klo[sb] + kk;
+        if (key <= W.khi[sb] && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
+          const u64 d = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
+          if (d) { sz = dir_size(d); off = dir_offset(d); }
+        }
+      }
+      W.ssize[sb][kk] = sz;
+      W.soff[sb][kk] = off;
+      const uint32_t reg = sz ? ((uint32_t)(off & 3) + sz + 3) & ~3u : 0u;
+      const uint32_t incl = wave_scan_add(reg, lane);
+      if (lane == 63) W.wsum[w] = incl;
+      __syncthreads();
+      uint32_t wb = base;
+      for (int j = 0; j < w; ++j) wb += W.wsum[j];
+      W.sstart[sb][kk] = wb + incl - reg + (uint32_t)(off & 3);
+      for (int j = 0; j < kWalkThreads / 64; ++j) base += W.wsum[j];
+      __syncthreads();
+    }
+    if (tid == 0) {
+      bool wide = false;
+      for (int s = 0; s < k.nsb; ++s) wide |= W.khi[s] - W.klo[s] + 1 > kWalkSlots;
+      W.total = (int)base;
+      W.staged = !wide && base <= (uint32_t)kWalkStreamBytes;
     }
   }
   __syncthreads();
-    const bool staged = D.staged;
+  const bool staged = W.staged;
+  if (staged) {
+    for (int i = tid; i < 8 * kWalkSlots; i += kWalkThreads) {
+      const int sb = i / kWalkSlots, kk = i % kWalkSlots;
+      const uint32_t sz = W.ssize[sb][kk];
+      if (!sz) continue;
+      const u64 off = W.soff[sb][kk];
+      const uint32_t* src = reinterpret_cast<const uint32_t*>(a.blob) + (off >> 2);
+      uint32_t* dst = W.sbuf + (W.sstart[sb][kk] >> 2);
+      const uint32_t nw = ((uint32_t)(off & 3) + sz + 3) >> 2;
+      uint32_t j = 0;
+      for (; j + 8 <= nw; j += 8) {
+        uint32_t t8[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) t8[q] = src[j + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) dst[j + q] = t8[q];
+      }
+      for (; j < nw; ++j) dst[j] = src[j];
+    }
+    if (tid < 4) W.sbuf[((uint32_t)W.total >> 2) + tid] = 0;
+  }
+  __syncthreads();
+  // ---- the walks ----
+  const int seg = lane >> 5, sl = lane & 31;
+  const int sb = 2 * w + seg;
+  const bool sb_ok = sb < k.nsb && ((a.mask >> sb) & 1) && W.khi[sb] >= W.klo[sb];
+  const int klo = sb_ok ? W.klo[sb] : 0, khi = sb_ok ? W.khi[sb] : -1;
+  const int nblk = sb_ok ? a.blk_n[sb] : 0;
+  const int sbi = sb < 8 ? sb : 0;
+  const int ngr = sb_ok ? (khi - klo) / 32 + 1 : 0;
+  const int ng = max(__builtin_amdgcn_readlane(ngr, 0), __builtin_amdgcn_readlane(ngr, 32));
+  const uint32_t* blocks = s_blocks + sbi * 128;
+  u64* xs = a.xs + ((size_t)bi * 8 + sbi) * 128 * npl;
+  for (int gi = 0; gi < ng; ++gi) {
+    const int g0 = khi - 32 * gi;
+    const bool gact = sb_ok && g0 >= klo;
+    const int key = g0 - sl;
+    const bool kact = gact && key >= klo;
+    const int kk = key - klo;
+    u64 dentry = 0;
     if (staged) {
-      // one thread per stream, aligned 32-bit words (all loads issued first)
-      const int sb = p.tid >> 7, kk = p.tid & 127;
-      const uint32_t sz = D.ssize[sb][kk];
-      if (sz) {
-        const u64 off = D.soff[sb][kk];
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.blob) + (off >> 2);
-        uint32_t* dst = D.sbuf + (D.sstart[sb][kk] >> 2);
-        const uint32_t nw = ((uint32_t)(off & 3) + sz + 3) >> 2;
-        uint32_t i = 0;
-        for (; i + 8 <= nw; i += 8) {
-          uint32_t t8[8];
-  #pragma unroll
-          for (int j = 0; j < 8; ++j) t8[j] = src[i + j];
-  #pragma unroll
-          for (int j = 0; j < 8; ++j) dst[i + j] = t8[j];
-        }
-        for (; i < nw; ++i) dst[i] = src[i];
-      }
-      if (p.tid < 4) D.sbuf[((uint32_t)D.total >> 2) + p.tid] = 0;
+      if (kact && kk < kWalkSlots && W.ssize[sbi][kk]) dentry = 1;
+    } else if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
+      dentry = a.dir[((size_t)sbi * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
     }
-    __syncthreads();
-  // the walk: wave sb walks subband sb
-  const int sb = p.w;
-  if (sb < k.nsb && ((a.mask >> sb) & 1) && D.khi[sb] >= D.klo[sb]) {
-    const int lane = p.lane;
-    const int klo = D.klo[sb], khi = D.khi[sb];
-    const int nblk = a.blk_n[sb];
-    const uint32_t* blocks = s_blocks + sb * 128;
-    u64* xs = XS + (size_t)sb * 128 * npl;
-    for (int g0 = khi; g0 >= klo; g0 -= 64) {
-      const int key = g0 - lane;
-      const bool kact = key >= klo;
-      const int kk = key - klo;
-      u64 dentry = 0;
-      if (staged) {
-        if (kact && kk < kMaxKeys && D.ssize[sb][kk]) dentry = 1;
-      } else if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
-        dentry = a.dir[((size_t)sb * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
+    const bool have = dentry != 0;
+    const uint32_t* w32 = staged ? W.sbuf : reinterpret_cast<const uint32_t*>(a.blob);
+    u64 pos = 0;
+    if (have) pos = staged ? (u64)W.sstart[sbi][kk] * 8 : dir_offset(dentry) * 8;
+    const int my_steps = gact ? nblk + min(32, g0 - klo + 1) : 0;
+    const int nsteps = max(__builtin_amdgcn_readlane(my_steps, 0), __builtin_amdgcn_readlane(my_steps, 32));
+    int n_out = 0, stop_out = 0;
+    int e_next = (sl == 0 && nblk > 0) ? W.E[sbi][0] : kSkip;
+    uint32_t bw_next = blocks[0];
+    for (int t = 0; t < nsteps; ++t) {
+      const int n_left = lane_prev_i(n_out);
+      const int s_left = lane_prev_i(stop_out);
+      const int b = t - sl;
+      const bool inb = b >= 0 && b < nblk;
+      const int e = e_next;
+      const uint32_t bwc = bw_next;
+      {
+        const int bn = b + 1;
+        const bool inn = bn >= 0 && bn < nblk;
+        e_next = inn ? W.E[sbi][bn] : kSkip;
+        bw_next = blocks[inn ? bn : 0];
       }
-      const bool have = dentry != 0;
-      const uint32_t* w32;
-      u64 pos;
-      if (staged) {
-        w32 = D.sbuf;
-        pos = have ? (u64)D.sstart[sb][kk] * 8 : 0;
-      } else {
-        w32 = reinterpret_cast<const uint32_t*>(a.blob);
-        pos = have ? dir_offset(dentry) * 8 : 0;
-      }
-      int n_out = 0, stop_out = 0;
-      const int nsteps = nblk + min(64, g0 - klo + 1);
-      int e_next = (-lane >= 0 && -lane < nblk) ? D.E[sb][0] : kSkip;
-      uint32_t bw_next = blocks[0];
-      for (int t = 0; t < nsteps; ++t) {
-        const int n_left = lane_prev_i(n_out);
-        const int s_left = lane_prev_i(stop_out);
-        const int b = t - lane;
-        const bool inb = b >= 0 && b < nblk;
-        const int e = e_next;
-        const uint32_t bwc = bw_next;
-        {  // prefetch the next step's block exponent and table word
-          const int bn = b + 1;
-          const bool inn = bn >= 0 && bn < nblk;
-          e_next = inn ? D.E[sb][bn] : kSkip;
-          bw_next = blocks[inn ? bn : 0];
+      if (kact && inb && e != kSkip) {
+        const int n_in = sl == 0 ? (gi == 0 ? 0 : W.nsave[sbi][b]) : n_left;
+        const int st_in = sl == 0 ? (gi == 0 ? 0 : W.ssave[sbi][b]) : s_left;
+        const int bp = key - kBpBiasDev - e;
+        const int stop = max(64 - k.endbp, 57 + a.etol - e);
+        n_out = n_in;
+        stop_out = st_in;
+        if (bp <= 63 && bp >= stop && !st_in) {
+          if (!have) {
+            stop_out = 1;  // missing chunk: the block stops decoding planes
+          } else {
+            const int nd = (((bwc >> 15) & 7) > 1) + (((bwc >> 18) & 7) > 1) + (((bwc >> 21) & 7) > 1);
+            int n = n_in;
+            const u64 X = decode_plane_win(w32, pos, n, 1 << (2 * nd));
+            n_out = n;
+            xs[b * npl + (63 - bp)] = X;
+            W.nbps[sbi][b] = (unsigned char)(64 - bp);
+          }
+        } else if (bp > 63) {
+          n_out = 0;
         }
-        if (kact && inb && e != kSkip) {
-          const int n_in = lane == 0 ? (g0 == khi ? 0 : D.nsave[sb][b]) : n_left;
-          const int st_in = lane == 0 ? (g0 == khi ? 0 : D.ssave[sb][b]) : s_left;
-          const int bp = key - kBpBiasDev - e;
-          const int stop = max(64 - k.endbp, 57 + a.etol - e);
-          n_out = n_in;
-          stop_out = st_in;
-          if (bp <= 63 && bp >= stop && !st_in) {
-            if (!have) {
-              stop_out = 1;  // missing chunk: the block stops decoding planes
-            } else {
-              const int nd = (((bwc >> 15) & 7) > 1) + (((bwc >> 18) & 7) > 1) + (((bwc >> 21) & 7) > 1);
-              int n = n_in;
-              const u64 X = decode_plane_win(w32, pos, n, 1 << (2 * nd));
-              n_out = n;
-              xs[b * npl + (63 - bp)] = X;
-              D.nbps[sb][b] = (unsigned char)(64 - bp);
-            }
-          } else if (bp > 63) {
-            n_out = 0;
-          }
-          if (lane == 63 || key == klo) {
-            D.nsave[sb][b] = (unsigned char)n_out;
-            D.ssave[sb][b] = (unsigned char)stop_out;
-          }
+        if (sl == 31 || key == klo) {
+          W.nsave[sbi][b] = (unsigned char)n_out;
+          W.ssave[sbi][b] = (unsigned char)stop_out;
         }
       }
     }
   }
   __syncthreads();
+  for (int i = tid; i < 8 * 128; i += kWalkThreads) a.nbps[(size_t)bi * 1024 + i] = (&W.nbps[0][0])[i];
 }
 
 // Step 3 for subband sb: plane words -> values in D.v.V (zeroed first; the
 // parent's interior already copied for subband 0 below the coarsest level).
 __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
                                                const uint32_t* s_blocks, const u64* XS) {
+  // XS: this brick's plane words [subband][128][npl]
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;

@@ -263,6 +263,22 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
   }
 }
 
+// Stream walks of full 32^3 bricks (dec32.hpp): one 256-thread workgroup per
+// brick, plane words and counts to the batch arrays a.xs / a.nbps.
+__global__ void __launch_bounds__(b32::kWalkThreads) k_walk_b32(const DecArgs* __restrict__ ap) {
+  const DecArgs& a = *ap;
+  __shared__ b32::WalkLds W;
+  __shared__ uint32_t s_blocks[8 * 128];
+  for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
+    const int s = i >> 7, b = i & 127;
+    s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
+  }
+  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+    b32::walk_wg(a, W, bi, s_blocks);
+    __syncthreads();
+  }
+}
+
 // Full 32^3 bricks of a 3D dataset (dec32.hpp): one brick per iteration,
 // 1024 threads, the brick on chip.
 __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restrict__ ap) {
@@ -282,15 +298,21 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
     const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
     w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.winv[s];
   }
-  u64* XS = reinterpret_cast<u64*>(a.scratch) + (size_t)blockIdx.x * (8 * 128 * 64);
+  const int npl = k.dtype ? 64 : 32;
   for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
     const b32::Pos p = b32::pos();
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
-    const u64 lin = a.lin[bi];
     const int dense = a.dense[bi];
+    const u64* XS = a.xs + (size_t)bi * 8 * 128 * npl;
     if (p.tid < 64) D.p3[p.tid] = a.perm.p3[p.tid];
     if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
-    b32::walk_brick(a, D, p, lin, dense, s_blocks, XS);
+    // exponents and decoded plane counts of the brick's blocks
+    {
+      const int sb = p.tid >> 7, b = p.tid & 127;
+      D.E[sb][b] = (sb < k.nsb && b < a.blk_n[sb]) ? (short)b32::block_exponent(a, s_blocks, sb, b, dense) : kSkip;
+      D.nbps[sb][b] = a.nbps[(size_t)bi * 1024 + p.tid];
+    }
+    __syncthreads();
     double c[b32::kN];
     for (int sb = 0; sb < 8; ++sb) {
       const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];

@@ -88,7 +88,9 @@ struct DecArgs {
   const u64* lin;        // [n] linear ids
   const int* dense;      // [n] dense ids (exponent rows)
   int nb;
-  double* scratch;       // per workgroup: a work volume (generic kernel) or plane words (b32)
+  double* scratch;       // per workgroup work volume (generic kernel)
+  u64* xs;               // b32: plane words of the batch [brick][subband][128 blocks][npl]
+  uint8_t* nbps;         // b32: planes decoded per block [brick][subband][128]
   const double* pgrid;   // level+1 decoded data (dense f64; subband-0 interiors of this level)
   int pgdims[3];
   double* cgrid;         // this level's decoded data when it is not the output level
