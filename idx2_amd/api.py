@@ -54,7 +54,8 @@ class Stats(ctypes.Structure):
 EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_encode",
            "idx2g_dataset_write", "idx2g_open", "idx2g_dataset_free", "idx2g_dataset_info",
            "idx2g_output_dims", "idx2g_decode", "idx2g_last_stats", "idx2g_dataset_chunk_bytes",
-           "idx2g_synth_field"]
+           "idx2g_synth_field", "idx2g_server_create", "idx2g_server_submit", "idx2g_server_wait",
+           "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint"]
 
 _lib = None
 
@@ -94,6 +95,13 @@ def lib():
         L.idx2g_dataset_chunk_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(i64)]
         L.idx2g_synth_field.argtypes = [vp, vp, ctypes.POINTER(i32), i32, ctypes.POINTER(ctypes.c_double),
                                         ctypes.c_uint64, i32, i32]
+        L.idx2g_server_create.argtypes = [ctypes.c_int, i64, ctypes.c_int, ctypes.POINTER(vp)]
+        L.idx2g_server_submit.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.c_int, ctypes.POINTER(i64)]
+        L.idx2g_server_wait.argtypes = [vp, i64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(Stats)]
+        L.idx2g_server_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)]
+        L.idx2g_server_destroy.argtypes = [vp]
+        L.idx2g_decode_footprint.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64)]
         _lib = L
     return _lib
 
@@ -242,3 +250,50 @@ class Dataset:
         _check(lib().idx2g_dataset_chunk_bytes(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                                ctypes.byref(n)))
         return buf
+
+
+class Server:
+    """Decode serving loop (idx2g_server): queued requests on resident
+    datasets, admitted against an HBM budget, several running at once."""
+
+    def __init__(self, device=0, hbm_budget=200 << 30, slots=4):
+        self._h = ctypes.c_void_p()
+        _check(lib().idx2g_server_create(device, int(hbm_budget), int(slots), ctypes.byref(self._h)))
+
+    def submit(self, ds, out_dev_ptr, **query):
+        """Queue a decode of `query` on `ds` into a device buffer; returns the request id."""
+        q = make_query(**query)
+        rid = ctypes.c_int64()
+        _check(lib().idx2g_server_submit(self._h, ds._h, ctypes.byref(q), ctypes.c_void_p(out_dev_ptr), 1,
+                                         ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        """Block until request `rid` is done: (latency_ms, queue_ms, stats)."""
+        lat, qms = ctypes.c_double(), ctypes.c_double()
+        st = Stats()
+        _check(lib().idx2g_server_wait(self._h, int(rid), ctypes.byref(lat), ctypes.byref(qms), ctypes.byref(st)))
+        return lat.value, qms.value, {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def info(self):
+        pb, pr, done = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()
+        _check(lib().idx2g_server_info(self._h, ctypes.byref(pb), ctypes.byref(pr), ctypes.byref(done)))
+        return dict(peak_bytes=pb.value, peak_running=pr.value, completed=done.value)
+
+    def close(self):
+        if self._h:
+            lib().idx2g_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def decode_footprint(ds, **query):
+    q = make_query(**query)
+    n = ctypes.c_int64()
+    _check(lib().idx2g_decode_footprint(ds._h, ctypes.byref(q), ctypes.byref(n)))
+    return n.value

@@ -443,3 +443,4 @@ static int encode_impl(idx2g_ctx* ctx, const idx2g_encode_params* p, const void*
 #include "files.inc"
 #include "decode.inc"
 #include "api.inc"
+#include "server.inc"

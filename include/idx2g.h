@@ -150,6 +150,33 @@ int idx2g_dataset_chunk_bytes(idx2g_dataset* ds, uint8_t* buf, int64_t* nbytes);
 int idx2g_synth_field(idx2g_ctx* ctx, void* dev_out, const int32_t dims[3], int32_t dtype,
                       const double* params, uint64_t noise_seed, int32_t z0, int32_t z1);
 
+/* ---- decode serving loop ----
+ * No counterpart in the reference (its decoder serves one query per call,
+ * Source/Core/idx2Decode.cpp:326-492, Source/idx2.h:43-44): a queue of decode
+ * requests on resident datasets, admitted in arrival order as soon as one of
+ * `slots` execution slots (one HIP stream each) is free and the request's
+ * HBM footprint fits `hbm_budget_bytes` (a request that exceeds the budget
+ * alone runs when nothing else holds memory).  Requests run concurrently;
+ * each result is the same as idx2g_decode's for the same query.  Datasets
+ * must outlive the server; they are only read. */
+typedef struct idx2g_server idx2g_server;
+int idx2g_server_create(int hip_device, int64_t hbm_budget_bytes, int slots, idx2g_server** out);
+/* Queue a decode of q (NULL: whole volume) into out (idx2g_output_dims
+ * bytes; device memory when out_is_device).  Returns at once. */
+int idx2g_server_submit(idx2g_server* s, idx2g_dataset* ds, const idx2g_query* q, void* out,
+                        int out_is_device, int64_t* request_id);
+/* Wait for a request; returns its decode error code.  latency_ms: submit to
+ * completion; queue_ms: submit to start.  Each id can be waited for once. */
+int idx2g_server_wait(idx2g_server* s, int64_t request_id, double* latency_ms, double* queue_ms,
+                      idx2g_stats* stats);
+/* Peak admitted bytes, peak concurrent requests, completed requests. */
+int idx2g_server_info(idx2g_server* s, int64_t* peak_bytes, int* peak_running, int64_t* completed);
+/* Drains the queue, then frees the server. */
+int idx2g_server_destroy(idx2g_server* s);
+/* The HBM footprint the server charges a request (output grid + working
+ * buffers of the decode). */
+int idx2g_decode_footprint(const idx2g_dataset* ds, const idx2g_query* q, int64_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif
