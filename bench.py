@@ -2,8 +2,8 @@
 
 A step is one encode of the whole grid (input resident in HBM -> chunks
 assembled in HBM) followed by one full-resolution decode (chunks in HBM ->
-grid in HBM).  value = raw input bytes / (encode + decode time) in GB/s.
-Prints one JSON line.
+grid in HBM).  value = raw input bytes x steps / wall time of the timed
+steps (GB/s).  Prints one JSON line.
 """
 import argparse
 import json
@@ -34,7 +34,7 @@ CONFIGS = {
 }
 # bounded CPU sample of the same workload for the reference binary
 CPU_SAMPLE = {
-    "c2": dict(dims=(256, 256, 256), dtype="float32", nlevels=3, tolerance=1e-4, brick=(32, 32, 32)),
+    "c2": dict(dims=(512, 512, 1024), dtype="float32", nlevels=3, tolerance=1e-4, brick=(32, 32, 32)),
     "c2b": dict(dims=(2048, 2048, 1), dtype="float64", nlevels=3, tolerance=1e-4, brick=(32, 32, 1)),
     "c1": dict(dims=(384, 384, 256), dtype="float64", nlevels=2, tolerance=1e-3, brick=(32, 32, 32)),
 }
@@ -71,6 +71,148 @@ def cpu_baseline(ctx, torch, cfg_name, seed):
                       f"{t1 - t0:.2f}s + serial decode {t2 - t1:.2f}s (includes file I/O)"}
 
 
+# Serving loop workload: resident datasets, Poisson arrivals at a fixed rate,
+# random ROI (edge 32-1024, clipped), downsampling U{0,1,2} per axis,
+# tolerance log-uniform in [1e-6, 1e-2], HBM admission budget 200 GB.
+SERVE_SETS = [((512, 512, 512), "float64"), ((1024, 1024, 1024), "float32"), ((2048, 2048, 512), "float32")]
+
+
+def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30):
+    import threading
+    from idx2_amd import api
+    sets = []
+    for i, (dims, dt) in enumerate(SERVE_SETS):
+        n = dims[0] * dims[1] * dims[2]
+        buf = torch.empty(n, dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+        ctx.synth_field(buf.data_ptr(), dims, dt, seed + 100 + i)
+        ds = api.Dataset.encode(ctx, dev_ptr=buf.data_ptr(), dims=dims, dtype=dt, name="s%d" % i, field="f",
+                                brick=(32, 32, 32), nlevels=3, tolerance=1e-6)
+        del buf
+        sets.append((ds, dims, dt))
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(seed + 7)
+    reqs = []
+    for _ in range(n_requests):
+        k = int(rng.integers(0, len(sets)))
+        ds, dims, dt = sets[k]
+        first, last = [], []
+        for d in range(3):
+            e = min(int(rng.integers(32, 1025)), dims[d])
+            f = int(rng.integers(0, dims[d] - e + 1))
+            first.append(f)
+            last.append(f + e - 1)
+        q = dict(first=tuple(first), last=tuple(last), downsampling=tuple(int(x) for x in rng.integers(0, 3, 3)),
+                 tolerance=float(10 ** rng.uniform(-6, -2)))
+        reqs.append((ds, dt, q, float(rng.exponential(1.0 / rate))))
+    server = api.Server(0, hbm_budget=budget, slots=slots)
+    lat, qms, out_bytes, read_bytes = [], [], 0, 0
+    pending = []
+    lock = threading.Condition()
+    done_flag = [False]
+
+    def waiter():
+        nonlocal out_bytes, read_bytes
+        while True:
+            with lock:
+                while not pending and not done_flag[0]:
+                    lock.wait()
+                if not pending:
+                    return
+                rid, buf = pending.pop(0)
+            l_, q_, st = server.wait(rid)
+            lat.append(l_)
+            qms.append(q_)
+            out_bytes += st["raw_bytes"]
+            read_bytes += st["chunk_bytes"]
+            del buf
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for ds, dt, q, gap in reqs:
+        t_next += gap
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        dims, nbytes = ds.output_dims(**q)
+        buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
+                          dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+        rid = server.submit(ds, buf.data_ptr(), **q)
+        with lock:
+            pending.append((rid, buf))
+            lock.notify()
+    with lock:
+        done_flag[0] = True
+        lock.notify()
+    th.join()
+    wall = time.perf_counter() - t0
+    info = server.info()
+    server.close()
+    for ds, _, _ in sets:
+        ds.free()
+    lat = np.array(lat)
+    return {"requests": n_requests, "rate_per_s": rate, "slots": slots, "hbm_budget_bytes": budget,
+            "datasets": [list(d) + [t] for d, t in SERVE_SETS],
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_queue_ms": round(float(np.mean(qms)), 3), "served_per_s": round(n_requests / wall, 2),
+            "output_gbps": round(out_bytes / wall / 1e9, 3), "stream_bytes_read": int(read_bytes),
+            "peak_concurrent": info["peak_running"], "peak_admitted_bytes": info["peak_bytes"]}
+
+
+PROGRESSIVE = dict(dims=(2048, 2048, 1024), dtype="float32", nlevels=3, tolerance=1e-6, brick=(32, 32, 32))
+
+
+def progressive_bench(ctx, torch, seed, n_roi=32):
+    """Encode 2048x2048x1024 float32 once; decode at downsampling (2,2,2),
+    (1,1,1), (0,0,0) x tolerance {1e-2, 1e-3, 1e-4, 1e-6}, full volume and
+    n_roi seeded random boxes (edge 64-512)."""
+    from idx2_amd import api
+    c = PROGRESSIVE
+    dims = c["dims"]
+    buf = torch.empty(dims[0] * dims[1] * dims[2], dtype=torch.float32, device="cuda")
+    ctx.synth_field(buf.data_ptr(), dims, "float32", seed)
+    ds = api.Dataset.encode(ctx, dev_ptr=buf.data_ptr(), dims=dims, dtype="float32", name="p", field="f",
+                            brick=c["brick"], nlevels=c["nlevels"], tolerance=c["tolerance"])
+    enc_ms = ctx.stats()["total_ms"]
+    del buf
+    rng = np.random.default_rng(seed + 11)
+    boxes = [None]
+    for _ in range(n_roi):
+        first, last = [], []
+        for d in range(3):
+            e = int(rng.integers(64, 513))
+            f = int(rng.integers(0, dims[d] - e + 1))
+            first.append(f)
+            last.append(f + e - 1)
+        boxes.append((tuple(first), tuple(last)))
+    rows = []
+    out = torch.empty(dims[0] * dims[1] * dims[2], dtype=torch.float32, device="cuda")
+    for dsmp in ((2, 2, 2), (1, 1, 1), (0, 0, 0)):
+        for tol in (1e-2, 1e-3, 1e-4, 1e-6):
+            for bx in boxes:
+                q = dict(downsampling=dsmp, tolerance=tol)
+                if bx is not None:
+                    q.update(first=bx[0], last=bx[1])
+                ds.decode(out_dev_ptr=out.data_ptr(), **q)
+                st = ctx.stats()
+                rows.append((dsmp, tol, bx is None, st["total_ms"], st["raw_bytes"], st["chunk_bytes"]))
+    ds.free()
+    summary = []
+    for dsmp in ((2, 2, 2), (1, 1, 1), (0, 0, 0)):
+        for tol in (1e-2, 1e-3, 1e-4, 1e-6):
+            full = [r for r in rows if r[0] == dsmp and r[1] == tol and r[2]][0]
+            rois = [r for r in rows if r[0] == dsmp and r[1] == tol and not r[2]]
+            roi_ms = np.array([r[3] for r in rois])
+            summary.append({"downsampling": list(dsmp), "tolerance": tol,
+                            "full_ms": round(full[3], 3), "full_out_gbps": round(full[4] / full[3] / 1e6, 2),
+                            "full_stream_bytes_read": int(full[5]),
+                            "roi_p50_ms": round(float(np.percentile(roi_ms, 50)), 3),
+                            "roi_out_gbps": round(sum(r[4] for r in rois) / roi_ms.sum() / 1e6, 2),
+                            "roi_stream_bytes_read": int(sum(r[5] for r in rois))})
+    return {"encode_ms": round(enc_ms, 2), "queries": len(rows), "sweep": summary}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +221,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=300, help="serving-loop requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving-loop arrival rate (requests/s)")
+    ap.add_argument("--progressive", action="store_true", help="also run the progressive decode sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -123,7 +268,7 @@ def main():
     dec_ms = np.array([d["total_ms"] for d in dec])
     enc_gbs = raw / (enc_ms.mean() / 1e3) / 1e9
     dec_gbs = raw / (dec_ms.mean() / 1e3) / 1e9
-    value = raw / ((enc_ms.mean() + dec_ms.mean()) / 1e3) / 1e9
+    value = raw * args.steps / wall / 1e9
     # dominant kernel: the level-0 brick kernel of encode or decode, whichever is longer
     dom_e = np.mean([e["dominant_ms"] for e in enc])
     dom_d = np.mean([d["dominant_ms"] for d in dec])
@@ -164,6 +309,11 @@ def main():
                      "kernel_ms": round(float(dom_ms), 3), "launches": dom_launches,
                      "algorithmic_bytes": int(dom_bytes)},
     }
+    del inp, out
+    if args.serve_requests > 0:
+        res["serving"] = serve_bench(ctx, torch, args.serve_requests, args.serve_rate, args.seed)
+    if args.progressive:
+        res["progressive"] = progressive_bench(ctx, torch, args.seed)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(ctx, torch, args.config, args.seed)
     print(json.dumps(res))

@@ -229,6 +229,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, co
       for (int s = 0; s < k.nsb; ++s) wide |= W.khi[s] - W.klo[s] + 1 > kWalkSlots;
       W.total = (int)base;
       W.staged = !wide && base <= (uint32_t)kWalkStreamBytes;
+      atomicAdd(&a.counters[1], (unsigned long long)base);  // stream bytes this query reads
     }
   }
   __syncthreads();
