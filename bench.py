@@ -40,6 +40,19 @@ CPU_SAMPLE = {
 }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the dominant kernel from its committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each calibrated against a plain device copy); None if absent."""
+    name = kernel.split()[0].split("::")[-1]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{name}_1024cube_f32.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            return int(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(ctx, torch, cfg_name, seed):
     """Time the reference idx2App (built from the reference sources into
     oracle/_ref) on a bounded sample of the same workload: encode + serial
@@ -224,11 +237,22 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300, help="serving-loop requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serving-loop arrival rate (requests/s)")
     ap.add_argument("--progressive", action="store_true", help="also run the progressive decode sweep")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run one plain device copy of MIB MiB (HBM counter calibration under rocprofv3) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import torch
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n}))
+        return
     from idx2_amd import api
 
     cfg = CONFIGS[args.config]
@@ -273,14 +297,15 @@ def main():
     dom_e = np.mean([e["dominant_ms"] for e in enc])
     dom_d = np.mean([d["dominant_ms"] for d in dec])
     if dom_e >= dom_d:
-        dom_name, dom_ms = "k_encode_level (level 0)", dom_e
+        dom_name, dom_ms = "idx2g::k_encode_b32 (level 0)", dom_e
         dom_bytes = np.mean([e["dominant_bytes"] for e in enc])
         dom_launches = 1
     else:
-        dom_name, dom_ms = "k_decode_level (level 0)", dom_d
+        dom_name, dom_ms = "idx2g::k_walk_b32 + idx2g::k_decode_b32 (level 0)", dom_d
         dom_bytes = np.mean([d["dominant_bytes"] for d in dec])
         dom_launches = int(np.mean([d["dominant_launches"] for d in dec]))
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    traffic = measured_traffic(dom_name) if args.config == "c2" else None
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -305,7 +330,7 @@ def main():
         "compressed_bytes": int(chunk_bytes),
         "max_abs_err": err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom_name,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom_name,
                      "kernel_ms": round(float(dom_ms), 3), "launches": dom_launches,
                      "algorithmic_bytes": int(dom_bytes)},
     }
