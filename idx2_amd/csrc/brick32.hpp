@@ -108,13 +108,14 @@ __device__ __forceinline__ double lift_lanes(double v, int i, int n) {
 template <int S>
 __device__ __forceinline__ void zcol_fwd(double (&c)[kN]) {
 #pragma unroll
-  for (int o = S; o < kN; o += 2 * S) c[o] = c[o] - (c[o - S] + c[o + S]) * 0.5;
+  for (int o = S; o < kN; o += 2 * S) { c[o] = c[o] - (c[o - S] + c[o + S]) * 0.5; pin(c[o]); }
 #pragma unroll
   for (int e = 0; e < kN; e += 2 * S) {
     double v = c[e];
     if (e > 0) v = v + c[e - S] * 0.25;
     if (e < 32) v = v + c[e + S] * 0.25;
     c[e] = v;
+    pin(c[e]);
   }
 }
 template <int S>
@@ -125,14 +126,15 @@ __device__ __forceinline__ void zcol_inv(double (&c)[kN]) {
     if (e > 0) v = v - c[e - S] * 0.25;
     if (e < 32) v = v - c[e + S] * 0.25;
     c[e] = v;
+    pin(c[e]);
   }
 #pragma unroll
-  for (int o = S; o < kN; o += 2 * S) c[o] = c[o] + (c[o - S] + c[o + S]) * 0.5;
+  for (int o = S; o < kN; o += 2 * S) { c[o] = c[o] + (c[o - S] + c[o + S]) * 0.5; pin(c[o]); }
 }
 __device__ __forceinline__ void zcol_ext(double (&c)[kN]) {
   c[32] = 2.0 * c[31] - c[30];
 #pragma unroll
-  for (int o = 1; o <= 29; o += 2) c[o] = c[o] - (c[o - 1] + c[o + 1]) * 0.5;
+  for (int o = 1; o <= 29; o += 2) { c[o] = c[o] - (c[o - 1] + c[o + 1]) * 0.5; pin(c[o]); }
   c[31] = 0.0;
 #pragma unroll
   for (int e = 0; e <= 30; e += 2) {
@@ -140,6 +142,7 @@ __device__ __forceinline__ void zcol_ext(double (&c)[kN]) {
     if (e > 0) v = v + c[e - 1] * 0.25;
     if (e < 30) v = v + c[e + 1] * 0.25;
     c[e] = v;
+    pin(c[e]);
   }
 }
 
@@ -327,6 +330,9 @@ __device__ __forceinline__ void ypass(double (&c)[kN], Lds& S, const Pos& p) {
         }
       }
     }
+#pragma unroll
+    for (int zc = 0; zc < kZC; ++zc)
+      if (z0 + zc < nz) pin(c[z0 + zc]);
   }
 }
 

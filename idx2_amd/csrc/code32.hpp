@@ -248,14 +248,10 @@ __device__ __forceinline__ void stage_cols(const double (&c)[kN], double* V, int
     if (k < dz) q[k * dx * dy] = c[HZ + 2 * k];
 }
 
-__device__ __forceinline__ void stage_subband(const double (&c)[kN], const Lds& S, double* V, const Pos& p, int hx,
-                                              int hy, int hz) {
+// Faces of subband (hx, hy, hz) from the x = 32 / y = 32 planes and the corner
+// line: x = 32 (hx = 0), y = 32 (hy = 0), corner.
+__device__ __forceinline__ void stage_faces(const Lds& S, double* V, const Pos& p, int hx, int hy, int hz) {
   const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
-  if ((p.x & 1) == hx && (p.y & 1) == hy) {
-    if (hz) stage_cols<1>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
-    else stage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
-  }
-  // faces: x = 32 (hx = 0), y = 32 (hy = 0), corner
   for (int t = p.tid; t < 17 * 16 * 2 + 17; t += kTh) {
     if (t < 272) {
       if (hx) continue;
@@ -273,6 +269,35 @@ __device__ __forceinline__ void stage_subband(const double (&c)[kN], const Lds& 
   }
 }
 
+// z-even subband (hx, hy, 0) into V: interior from the register columns.
+__device__ __forceinline__ void stage_subband_even(const double (&c)[kN], const Lds& S, double* V, const Pos& p, int hx,
+                                                   int hy) {
+  const int dx = 17 - hx, dy = 17 - hy;
+  if ((p.x & 1) == hx && (p.y & 1) == hy) stage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, 17);
+  stage_faces(S, V, p, hx, hy, 0);
+}
+
+// The z-odd half of the register columns, parked in the workgroup's scratch
+// volume subband by subband: Z1[((hx + 2 hy) * 16 + k) * 256 + j * 16 + i]
+// holds column (2i + hx, 2j + hy) at z = 2k + 1.
+constexpr int kOddOff = 16384;  // doubles into the scratch volume
+__device__ __forceinline__ void park_odd(const double (&c)[kN], double* Z1, const Pos& p) {
+  double* q = Z1 + ((p.x & 1) + 2 * (p.y & 1)) * 16 * 256 + (p.y >> 1) * 16 + (p.x >> 1);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) q[k * 256] = c[2 * k + 1];
+}
+
+// z-odd subband (hx, hy, 1) into V from the parked columns.
+__device__ __forceinline__ void stage_subband_odd(const double* Z1, const Lds& S, double* V, const Pos& p, int hx,
+                                                  int hy) {
+  const int dx = 17 - hx, dy = 17 - hy;
+  const double* src = Z1 + (hx + 2 * hy) * 16 * 256;
+  for (int t = p.tid; t < 16 * 256; t += kTh) {
+    const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
+    V[(k * dy + j) * dx + i] = src[t];
+  }
+  stage_faces(S, V, p, hx, hy, 1);
+}
 
 __device__ __forceinline__ uint32_t& lno_at(CodeLds& C, int npl, int b, int pp) { return C.pool[b * npl + pp]; }
 __device__ __forceinline__ u64* plane_at(CodeLds& C, u64* XG, int npl, int b, int pp) {
@@ -308,6 +333,7 @@ __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bo
 
 // Block stage of one subband staged in C.v.V (dims dx, dy, ...): per block
 // exponent, planes and their (length, N); 4 blocks per wave.
+template <int DT>
 __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint32_t* blocks, int nblk, CodeLds& C,
                             const Pos& p, int dense, u64* XG) {
   const BrickConst& k = a.k;
@@ -338,7 +364,7 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
     for (int i0 = 0; i0 < 4; ++i0) {
       const bool ok = rowok && i0 < n0;
       v[i0] = ok ? C.v.V[vbase + i0 * s0] : 0.0;
-      const int ev = k.dtype ? exponent_f64(fabs(v[i0])) : exponent_f32((float)fabs(v[i0]));
+      const int ev = DT ? exponent_f64(fabs(v[i0])) : exponent_f32((float)fabs(v[i0]));
       if (ok) emx = max(emx, ev);
     }
     emx = row_max(emx);
@@ -384,8 +410,8 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
       C.np[b] = (unsigned char)np;
       if (!skip) {
         // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
-        const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (k.dtype ? 2 : 1);
-        if (k.dtype) {
+        const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (DT ? 2 : 1);
+        if (DT) {
           const uint16_t e16 = (uint16_t)(emax + 1023);
           a.exps[row] = (uint8_t)(e16 & 0xff);
           a.exps[row + 1] = (uint8_t)(e16 >> 8);
@@ -403,7 +429,7 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
     const int np_s[4] = {__builtin_amdgcn_readlane(np, 0), __builtin_amdgcn_readlane(np, 16),
                          __builtin_amdgcn_readlane(np, 32), __builtin_amdgcn_readlane(np, 48)};
-    if (k.dtype == 0) {
+    if (DT == 0) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
         const u64 uA = coef_u(T + (2 * pr) * 64, C, nd_s[2 * pr], lane);
@@ -431,12 +457,141 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
 
 // Code every subband of the brick held in c (register columns) and S
 // (faces); writes exponent rows, streams into the arena, entries, headers.
+// One subband of the brick: stage into V, block stage, key offsets, emission.
+template <int DT, bool kOdd>
+__device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const double (&c)[kN], const double* Z1, Lds& S,
+                                             CodeLds& C, const Pos& p, int dense, u64* XG, const uint32_t* s_blocks,
+                                             ArenaCursor& cur) {
+  const BrickConst& k = a.k;
+  constexpr int npl = DT ? 64 : 32;
+  const int lane = p.lane;
+  const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
+  const int dx = 17 - hx, dy = 17 - hy;
+  const int nblk = a.blk_n[sb];
+  const uint32_t* blocks = s_blocks + sb * 128;
+  if (kOdd) stage_subband_odd(Z1, S, C.v.V, p, hx, hy);
+  else stage_subband_even(c, S, C.v.V, p, hx, hy);
+  if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; C.over = 0; }
+  __syncthreads();
+  block_stage<DT>(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
+  __syncthreads();
+  // ---- 4. per-key offsets over the blocks (Morton order) ----
+  const int klo = C.klo, khi = C.khi;
+  const int nk = khi >= klo ? khi - klo + 1 : 0;
+  if (nk > kMaxKeys) {
+    if (p.tid == 0) {
+      atomicOr(&a.counters[2], 1ull);
+      a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0};
+    }
+    __syncthreads();
+    return;
+  }
+  for (int kk = p.w; kk < nk; kk += kTh / 64) {
+    const int key = klo + kk;
+    uint32_t carry = 0;
+    int first = -1;
+    for (int c0 = 0; c0 < nblk; c0 += 64) {
+      const int b = c0 + lane;
+      const int e = b < nblk ? C.E[b] : kSkip;
+      const int pp = 63 - (key - kBpBiasDev - e);
+      const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
+      const uint32_t len = coded ? (lno_at(C, npl, b, pp) & 0xff) : 0u;
+      const uint32_t incl = wave_scan_add(len, lane);
+      if (coded) lno_at(C, npl, b, pp) |= (incl - len + carry) << 15;
+      const u64 m = __ballot(coded);
+      if (first < 0 && m) first = c0 + __ffsll((long long)m) - 1;
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+      }
+      if (lane == 0) {
+        C.kbits[kk] = carry;
+      C.kfirst[kk] = (short)first;
+    }
+  }
+  __syncthreads();
+  // byte layout of the key streams + arena reservation (wave 0)
+  if (p.w == 0) {
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < nk; c0 += 64) {
+      const int kk = c0 + lane;
+      const uint32_t by = kk < nk ? (C.kbits[kk] + 7) >> 3 : 0u;
+      const uint32_t incl = wave_scan_add(by, lane);
+      if (kk < nk) C.kbyte[kk] = incl - by + carry;
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+    if (lane == 0) {
+      const unsigned long long alloc = (carry + 3ull) & ~3ull;
+      if (alloc > cur.aleft) {
+        const unsigned long long take = alloc > kArenaChunk ? alloc : kArenaChunk;
+        cur.abase = atomicAdd(&a.counters[0], take);
+        cur.aleft = take;
+      }
+      if ((unsigned long long)nk > cur.eleft) {
+        const unsigned long long take = (unsigned long long)nk > kEntryChunk ? (unsigned long long)nk : kEntryChunk;
+        cur.ebase = atomicAdd(&a.counters[1], take);
+        cur.eleft = take;
+      }
+      const unsigned long long bse = cur.abase, ebs = cur.ebase;
+      cur.abase += alloc;
+      cur.aleft -= alloc;
+      cur.ebase += nk;
+      cur.eleft -= nk;
+      if (carry) atomicAdd(&a.counters[3], (unsigned long long)carry);
+      const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
+      if (over) atomicOr(&a.counters[2], 2ull);
+      C.base = bse;
+      C.ebase = ebs;
+      C.total = carry;
+      C.over = over;
+      SbHdr h;
+      h.klo = klo;
+      h.nkeys = over ? 0 : nk;
+      h.ebase = ebs;
+      a.hdrs[(size_t)dense * k.nsb + sb] = h;
+    }
+  }
+  __syncthreads();
+  if (C.over) return;
+  const unsigned long long base = C.base;
+  const uint32_t nwords = (C.total + 3) >> 2;
+  const bool staged = nwords <= (uint32_t)kStreamWords;
+  uint32_t* gwords = (uint32_t*)(a.arena + base);
+  for (int kk = p.tid; kk < nk; kk += kTh) {
+    SbEntry e;
+    e.off = base + C.kbyte[kk];
+    e.size = (C.kbits[kk] + 7) >> 3;
+    e.first_block = (uint16_t)(C.kfirst[kk] < 0 ? 0xffff : C.kfirst[kk]);
+    e.key = (uint16_t)(klo + kk);
+    a.entries[C.ebase + kk] = e;
+  }
+  // ---- 5. emission ----
+  uint32_t* words = staged ? C.v.stream : gwords;
+  for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
+  if (!staged) __threadfence();
+  __syncthreads();
+  for (int t = p.tid; t < nblk * npl; t += kTh) {
+    const int b = t / npl, pp = t - b * npl;
+    const int e = C.E[b];
+    if (e == kSkip || pp >= C.np[b]) continue;
+    const uint32_t bw = blocks[b];
+    const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
+    const uint32_t lno = lno_at(C, npl, b, pp);
+    u64 lo, hi;
+    plane_bits(*plane_at(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
+    const int kk = (63 - pp) + e + kBpBiasDev - klo;
+    const u64 pos = (u64)C.kbyte[kk] * 8 + (lno >> 15);
+    if (staged) or_bits128<true>(words, pos, lo, hi);
+    else or_bits128<false>(words, pos, lo, hi);
+  }
+  __syncthreads();
+  if (staged)
+    for (uint32_t i = p.tid; i < nwords; i += kTh) gwords[i] = words[i];
+  __syncthreads();
+}
+
+template <int DT>
 __device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, CodeLds& C, const Pos& p,
                                            int dense, u64* XG, const uint32_t* s_blocks, ArenaCursor& cur) {
   const BrickConst& k = a.k;
-  const int npl = k.dtype ? 64 : 32;
-  const int lane = p.lane, q = lane >> 4, r = lane & 15;
-  i64* T = &C.T[p.w][0][0];
   if (p.tid < 64) C.p3[p.tid] = a.perm.p3[p.tid];
   if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
   if (p.tid < 256) {
@@ -448,128 +603,14 @@ __device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[k
     }
     C.e8[p.tid] = (uint16_t)e;
   }
-  for (int sb = 0; sb < 8; ++sb) {
-    const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
-    const int dx = 17 - hx, dy = 17 - hy;
-    const int nblk = a.blk_n[sb];
-    const uint32_t* blocks = s_blocks + sb * 128;
-    stage_subband(c, S, C.v.V, p, hx, hy, hz);
-    if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; C.over = 0; }
-    __syncthreads();
-    block_stage(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
-    __syncthreads();
-    // ---- 4. per-key offsets over the blocks (Morton order) ----
-    const int klo = C.klo, khi = C.khi;
-    const int nk = khi >= klo ? khi - klo + 1 : 0;
-    if (nk > kMaxKeys) {
-      if (p.tid == 0) {
-        atomicOr(&a.counters[2], 1ull);
-        a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0};
-      }
-      __syncthreads();
-      continue;
-    }
-    for (int kk = p.w; kk < nk; kk += kTh / 64) {
-      const int key = klo + kk;
-      uint32_t carry = 0;
-      int first = -1;
-      for (int c0 = 0; c0 < nblk; c0 += 64) {
-        const int b = c0 + lane;
-        const int e = b < nblk ? C.E[b] : kSkip;
-        const int pp = 63 - (key - kBpBiasDev - e);
-        const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
-        const uint32_t len = coded ? (lno_at(C, npl, b, pp) & 0xff) : 0u;
-        const uint32_t incl = wave_scan_add(len, lane);
-        if (coded) lno_at(C, npl, b, pp) |= (incl - len + carry) << 15;
-        const u64 m = __ballot(coded);
-        if (first < 0 && m) first = c0 + __ffsll((long long)m) - 1;
-        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        }
-        if (lane == 0) {
-          C.kbits[kk] = carry;
-        C.kfirst[kk] = (short)first;
-      }
-    }
-    __syncthreads();
-    // byte layout of the key streams + arena reservation (wave 0)
-    if (p.w == 0) {
-      uint32_t carry = 0;
-      for (int c0 = 0; c0 < nk; c0 += 64) {
-        const int kk = c0 + lane;
-        const uint32_t by = kk < nk ? (C.kbits[kk] + 7) >> 3 : 0u;
-        const uint32_t incl = wave_scan_add(by, lane);
-        if (kk < nk) C.kbyte[kk] = incl - by + carry;
-        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-      }
-      if (lane == 0) {
-        const unsigned long long alloc = (carry + 3ull) & ~3ull;
-        if (alloc > cur.aleft) {
-          const unsigned long long take = alloc > kArenaChunk ? alloc : kArenaChunk;
-          cur.abase = atomicAdd(&a.counters[0], take);
-          cur.aleft = take;
-        }
-        if ((unsigned long long)nk > cur.eleft) {
-          const unsigned long long take = (unsigned long long)nk > kEntryChunk ? (unsigned long long)nk : kEntryChunk;
-          cur.ebase = atomicAdd(&a.counters[1], take);
-          cur.eleft = take;
-        }
-        const unsigned long long bse = cur.abase, ebs = cur.ebase;
-        cur.abase += alloc;
-        cur.aleft -= alloc;
-        cur.ebase += nk;
-        cur.eleft -= nk;
-        if (carry) atomicAdd(&a.counters[3], (unsigned long long)carry);
-        const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
-        if (over) atomicOr(&a.counters[2], 2ull);
-        C.base = bse;
-        C.ebase = ebs;
-        C.total = carry;
-        C.over = over;
-        SbHdr h;
-        h.klo = klo;
-        h.nkeys = over ? 0 : nk;
-        h.ebase = ebs;
-        a.hdrs[(size_t)dense * k.nsb + sb] = h;
-      }
-    }
-    __syncthreads();
-    if (C.over) continue;
-    const unsigned long long base = C.base;
-    const uint32_t nwords = (C.total + 3) >> 2;
-    const bool staged = nwords <= (uint32_t)kStreamWords;
-    uint32_t* gwords = (uint32_t*)(a.arena + base);
-    for (int kk = p.tid; kk < nk; kk += kTh) {
-      SbEntry e;
-      e.off = base + C.kbyte[kk];
-      e.size = (C.kbits[kk] + 7) >> 3;
-      e.first_block = (uint16_t)(C.kfirst[kk] < 0 ? 0xffff : C.kfirst[kk]);
-      e.key = (uint16_t)(klo + kk);
-      a.entries[C.ebase + kk] = e;
-    }
-    // ---- 5. emission ----
-    uint32_t* words = staged ? C.v.stream : gwords;
-    for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
-    if (!staged) __threadfence();
-    __syncthreads();
-    for (int t = p.tid; t < nblk * npl; t += kTh) {
-      const int b = t / npl, pp = t - b * npl;
-      const int e = C.E[b];
-      if (e == kSkip || pp >= C.np[b]) continue;
-      const uint32_t bw = blocks[b];
-      const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
-      const uint32_t lno = lno_at(C, npl, b, pp);
-      u64 lo, hi;
-      plane_bits(*plane_at(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
-      const int kk = (63 - pp) + e + kBpBiasDev - klo;
-      const u64 pos = (u64)C.kbyte[kk] * 8 + (lno >> 15);
-      if (staged) or_bits128<true>(words, pos, lo, hi);
-      else or_bits128<false>(words, pos, lo, hi);
-    }
-    __syncthreads();
-    if (staged)
-      for (uint32_t i = p.tid; i < nwords; i += kTh) gwords[i] = words[i];
-    __syncthreads();
-  }
+  // the z-odd half leaves the registers first; the z-even subbands are coded
+  // from the registers, the z-odd ones from the parked copy
+  double* Z1 = reinterpret_cast<double*>(XG) + kOddOff;
+  park_odd(c, Z1, p);
+  for (int sb = 0; sb < 8; ++sb)
+    if (k.sb[sb].f[2] == 0) code_subband<DT, false>(a, sb, c, Z1, S, C, p, dense, XG, s_blocks, cur);
+  for (int sb = 0; sb < 8; ++sb)
+    if (k.sb[sb].f[2] == 1) code_subband<DT, true>(a, sb, c, Z1, S, C, p, dense, XG, s_blocks, cur);
 }
 
 }  // namespace b32

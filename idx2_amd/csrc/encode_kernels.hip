@@ -345,6 +345,7 @@ __global__ void __launch_bounds__(kThreads) k_encode_level(const EncArgs* __rest
 // Full 32^3 bricks of a 3D grid: the transform runs on chip (brick32.hpp),
 // the transformed brick is handed to the coding stages through the
 // workgroup's scratch volume.
+template <int DT>
 __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restrict__ ap) {
   const EncArgs& a = *ap;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -404,7 +405,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
       for (int z = 0; z < 16; ++z) dst[po + (unsigned)z * pplane] = c[2 * z];
     }
     b32::weigh(c, S, p, w8);
-    b32::code_brick(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
+    b32::code_brick<DT>(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
                     reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * k.nvox), s_blocks, cur);
     __syncthreads();
   }

@@ -133,7 +133,8 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
   const size_t lds = enc_lds_bytes(npl);
   const size_t lds32 = std::max(sizeof(b32::Lds), offsetof(b32::Lds, u) + sizeof(b32::CodeLds));
   HIPCHK(hipFuncSetAttribute((const void*)k_encode_level, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  HIPCHK(hipFuncSetAttribute((const void*)k_encode_b32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32));
+  const void* kb32 = L.dtype ? (const void*)k_encode_b32<1> : (const void*)k_encode_b32<0>;
+  HIPCHK(hipFuncSetAttribute(kb32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds32));
   HIPCHK(hipMemsetAsync(B.counters, 0, 4 * sizeof(unsigned long long), st));
   int sbne[3], gd[3];
   for (int d = 0; d < 3; ++d) {
@@ -220,7 +221,10 @@ static int run_encode_levels(idx2g_ctx* ctx, idx2g_dataset* ds, const void* d_in
       a.nb = (int)full.size();
       const EncArgs* da;
       TRY(stage_args(ctx, a, &da));
-      hipLaunchKernelGGL(k_encode_b32, dim3(std::min(a.nb, grid_max)), dim3(b32::kTh), lds32, st, da);
+      if (L.dtype)
+        hipLaunchKernelGGL(k_encode_b32<1>, dim3(std::min(a.nb, grid_max)), dim3(b32::kTh), lds32, st, da);
+      else
+        hipLaunchKernelGGL(k_encode_b32<0>, dim3(std::min(a.nb, grid_max)), dim3(b32::kTh), lds32, st, da);
       HIPCHK(hipGetLastError());
     }
     if (l == 0 && !full.empty()) HIPCHK(hipEventRecord(ctx->ev[4], st));
