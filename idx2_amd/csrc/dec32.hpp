@@ -118,13 +118,10 @@ __device__ __forceinline__ void unstage_cols(double (&c)[kN], const double* V, i
     if (k < dz) c[HZ + 2 * k] = q[k * dx * dy];
 }
 
-__device__ __forceinline__ void unstage_subband(double (&c)[kN], Lds& S, const double* V, const Pos& p, int hx, int hy,
-                                                int hz) {
+// Faces of subband (hx, hy, hz) from V back to the x = 32 / y = 32 planes and
+// the corner line.
+__device__ __forceinline__ void unstage_faces(Lds& S, const double* V, const Pos& p, int hx, int hy, int hz) {
   const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
-  if ((p.x & 1) == hx && (p.y & 1) == hy) {
-    if (hz) unstage_cols<1>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
-    else unstage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, dz);
-  }
   for (int t = p.tid; t < 17 * 16 * 2 + 17; t += kTh) {
     if (t < 272) {
       if (hx) continue;
@@ -140,6 +137,33 @@ __device__ __forceinline__ void unstage_subband(double (&c)[kN], Lds& S, const d
       if (k < dz) S.cc[hz + 2 * k] = V[(k * dy + 16) * dx + 16];
     }
   }
+}
+
+// z-even subband (hx, hy, 0) from V into the register columns.
+__device__ __forceinline__ void unstage_subband_even(double (&c)[kN], Lds& S, const double* V, const Pos& p, int hx,
+                                                     int hy) {
+  const int dx = 17 - hx, dy = 17 - hy;
+  if ((p.x & 1) == hx && (p.y & 1) == hy) unstage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, 17);
+  unstage_faces(S, V, p, hx, hy, 0);
+}
+
+// z-odd subband (hx, hy, 1) from V into the workgroup's parking area
+// (Z1[((hx + 2 hy) * 16 + k) * 256 + j * 16 + i] = column (2i + hx, 2j + hy)
+// at z = 2k + 1); the columns take them after every subband is decoded.
+__device__ __forceinline__ void park_subband_odd(double* Z1, Lds& S, const double* V, const Pos& p, int hx, int hy) {
+  const int dx = 17 - hx, dy = 17 - hy;
+  double* dst = Z1 + (hx + 2 * hy) * 16 * 256;
+  for (int t = p.tid; t < 16 * 256; t += kTh) {
+    const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
+    dst[t] = V[(k * dy + j) * dx + i];
+  }
+  unstage_faces(S, V, p, hx, hy, 1);
+}
+
+__device__ __forceinline__ void unpark_odd(double (&c)[kN], const double* Z1, const Pos& p) {
+  const double* q = Z1 + ((p.x & 1) + 2 * (p.y & 1)) * 16 * 256 + (p.y >> 1) * 16 + (p.x >> 1);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) c[2 * k + 1] = q[k * 256];
 }
 
 // Steps 1-2, one 256-thread workgroup per brick (k_walk_b32): stream
@@ -335,11 +359,12 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, co
 
 // Step 3 for subband sb: plane words -> values in D.v.V (zeroed first; the
 // parent's interior already copied for subband 0 below the coarsest level).
+template <int DT>
 __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
                                                const uint32_t* s_blocks, const u64* XS) {
   // XS: this brick's plane words [subband][128][npl]
   const BrickConst& k = a.k;
-  const int npl = k.dtype ? 64 : 32;
+  constexpr int npl = DT ? 64 : 32;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   const int nblk = a.blk_n[sb];
   const uint32_t* blocks = s_blocks + sb * 128;
@@ -366,7 +391,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
     // plane words -> coefficients (bit transpose), inverse order, negabinary
-    if (k.dtype == 0) {
+    if (DT == 0) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
         const int slot = 2 * pr + (lane >> 5);

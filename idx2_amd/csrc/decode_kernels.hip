@@ -279,8 +279,36 @@ __global__ void __launch_bounds__(b32::kWalkThreads) k_walk_b32(const DecArgs* _
   }
 }
 
+// One subband of a brick: zero V, parent interior (subband 0), inverse block
+// stage of the walked planes.
+template <int DT>
+__device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, const int (&b3)[3], b32::DecLds32& D,
+                                                   const b32::Pos& p, const uint32_t* s_blocks, const u64* XS) {
+  const BrickConst& k = a.k;
+  const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
+  const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+  for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
+  __syncthreads();
+  if ((a.mask >> sb) & 1) {
+    if (sb == 0 && !a.coarsest) {
+      // parent's decoded data: subband-0 interior (CopyExtentGrid, Volume.h:1153-1173)
+      for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
+        const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
+        D.v.V[(z * 17 + y) * 17 + x] =
+            a.pgrid[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
+      }
+      __syncthreads();
+    }
+    b32::inverse_blocks<DT>(a, sb, dx, dy, D, p, s_blocks, XS);
+  }
+  __syncthreads();
+}
+
 // Full 32^3 bricks of a 3D dataset (dec32.hpp): one brick per iteration,
-// 1024 threads, the brick on chip.
+// 1024 threads, the brick on chip.  The z-odd subbands are decoded first and
+// parked in the workgroup's scratch area, so that only the z-even half of the
+// register columns is live while the z-even subbands are decoded.
+template <int DT>
 __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restrict__ ap) {
   const DecArgs& a = *ap;
   extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -298,7 +326,8 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
     const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
     w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.winv[s];
   }
-  const int npl = k.dtype ? 64 : 32;
+  constexpr int npl = DT ? 64 : 32;
+  double* Z1 = a.scratch + (size_t)blockIdx.x * k.nvox;
   for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
     const b32::Pos p = b32::pos();
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
@@ -313,28 +342,20 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
       D.nbps[sb][b] = a.nbps[(size_t)bi * 1024 + p.tid];
     }
     __syncthreads();
-    double c[b32::kN];
     for (int sb = 0; sb < 8; ++sb) {
-      const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
-      const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
-      for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
-      __syncthreads();
-      if ((a.mask >> sb) & 1) {
-        if (sb == 0 && !a.coarsest) {
-          // parent's decoded data: subband-0 interior (CopyExtentGrid, Volume.h:1153-1173)
-          for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
-            const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
-            D.v.V[(z * 17 + y) * 17 + x] =
-                a.pgrid[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
-          }
-          __syncthreads();
-        }
-        b32::inverse_blocks(a, sb, dx, dy, D, p, s_blocks, XS);
-      }
-      __syncthreads();
-      b32::unstage_subband(c, S, D.v.V, p, hx, hy, hz);
+      if (k.sb[sb].f[2] != 1) continue;
+      decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS);
+      b32::park_subband_odd(Z1, S, D.v.V, p, k.sb[sb].f[0], k.sb[sb].f[1]);
       __syncthreads();
     }
+    double c[b32::kN];
+    for (int sb = 0; sb < 8; ++sb) {
+      if (k.sb[sb].f[2] != 0) continue;
+      decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS);
+      b32::unstage_subband_even(c, S, D.v.V, p, k.sb[sb].f[0], k.sb[sb].f[1]);
+      __syncthreads();
+    }
+    b32::unpark_odd(c, Z1, p);
     b32::weigh(c, S, p, w8);
     b32::inverse(c, S, p);
     if (a.out) {
@@ -357,7 +378,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
           const int gp = b3[2] * a.B3[2] + z * a.bstride;
           if (gp >= a.og_f[2] && gp <= olast && (gp - a.og_f[2]) % a.og_s[2] == 0) {
             const size_t oi = (size_t)((gp - a.og_f[2]) / a.og_s[2]) * plane + row;
-            if (k.dtype) ((double*)a.out)[oi] = c[z];
+            if (DT) ((double*)a.out)[oi] = c[z];
             else ((float*)a.out)[oi] = (float)c[z];
           }
         }
