@@ -55,7 +55,8 @@ EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_e
            "idx2g_dataset_write", "idx2g_open", "idx2g_dataset_free", "idx2g_dataset_info",
            "idx2g_output_dims", "idx2g_decode", "idx2g_last_stats", "idx2g_dataset_chunk_bytes",
            "idx2g_synth_field", "idx2g_server_create", "idx2g_server_submit", "idx2g_server_wait",
-           "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint"]
+           "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint", "idx2g_encode_batch",
+           "idx2g_decode_batch", "idx2g_validate_params"]
 
 _lib = None
 
@@ -102,6 +103,11 @@ def lib():
         L.idx2g_server_info.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64)]
         L.idx2g_server_destroy.argtypes = [vp]
         L.idx2g_decode_footprint.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64)]
+        L.idx2g_validate_params.argtypes = [ctypes.POINTER(EncodeParams)]
+        L.idx2g_encode_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(EncodeParams), ctypes.POINTER(vp),
+                                         ctypes.c_int, ctypes.POINTER(vp)]
+        L.idx2g_decode_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(Query),
+                                         ctypes.POINTER(vp), ctypes.c_int]
         _lib = L
     return _lib
 
@@ -163,16 +169,17 @@ class Dataset:
         self.ctx = ctx
         self._h = handle
 
-    @classmethod
-    def encode(cls, ctx, data=None, *, dev_ptr=None, dims=None, dtype=None, name="data", field="field",
-               brick=(32, 32, 32), nlevels=1, tolerance=1e-3, bricks_per_chunk=4096, chunks_per_file=64,
-               files_per_dir=64):
-        """Encode a numpy array of shape (nz, ny, nx), or a device buffer."""
+    @staticmethod
+    def _params(data=None, *, dev_ptr=None, dims=None, dtype=None, name="data", field="field",
+                brick=(32, 32, 32), nlevels=1, tolerance=1e-3, bricks_per_chunk=4096, chunks_per_file=64,
+                files_per_dir=64):
+        """(EncodeParams, input pointer, input is on the device, kept objects)."""
         p = EncodeParams()
-        p.name = name.encode()
-        p.field = field.encode()
+        keep = [name.encode(), field.encode()]
+        p.name, p.field = keep[0], keep[1]
         if data is not None:
             data = np.ascontiguousarray(data)
+            keep.append(data)
             nz, ny, nx = data.shape
             dims = (nx, ny, nz)
             dtype = "float64" if data.dtype == np.float64 else "float32"
@@ -189,9 +196,39 @@ class Dataset:
         p.chunks_per_file = chunks_per_file
         p.files_per_dir = files_per_dir
         p.bit_planes_per_chunk = 1
+        return p, ptr, is_dev, keep
+
+    @classmethod
+    def encode(cls, ctx, data=None, **kw):
+        """Encode a numpy array of shape (nz, ny, nx), or a device buffer (dev_ptr=, dims=, dtype=)."""
+        p, ptr, is_dev, keep = cls._params(data, **kw)
         h = ctypes.c_void_p()
         _check(lib().idx2g_encode(ctx._h, ctypes.byref(p), ptr, is_dev, ctypes.byref(h)))
+        del keep
         return cls(ctx, h)
+
+    @classmethod
+    def encode_batch(cls, ctx, fields):
+        """Encode several fields in one pass (idx2g_encode_batch).  fields: list of
+        dicts of Dataset.encode arguments, all with `data` (host arrays) or all
+        with `dev_ptr` (device buffers).  Returns one Dataset per field."""
+        n = len(fields)
+        ps = (EncodeParams * n)()
+        ptrs = (ctypes.c_void_p * n)()
+        keep, devs = [], set()
+        for i, f in enumerate(fields):
+            f = dict(f)
+            p, ptr, is_dev, k = cls._params(f.pop("data", None), **f)
+            ps[i] = p
+            ptrs[i] = ptr
+            devs.add(is_dev)
+            keep.append(k)
+        if len(devs) != 1:
+            raise ValueError("encode_batch: inputs must be all host arrays or all device buffers")
+        outs = (ctypes.c_void_p * n)()
+        _check(lib().idx2g_encode_batch(ctx._h, n, ps, ptrs, devs.pop(), outs))
+        del keep
+        return [cls(ctx, ctypes.c_void_p(outs[i])) for i in range(n)]
 
     @classmethod
     def open(cls, ctx, idx2_path, in_dir=None):
@@ -250,6 +287,39 @@ class Dataset:
         _check(lib().idx2g_dataset_chunk_bytes(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                                ctypes.byref(n)))
         return buf
+
+
+def validate_params(**kw):
+    """Error code idx2g_encode would return for these Dataset.encode keyword
+    arguments (dims=, dtype=, brick=, nlevels=, ...), checked on the host."""
+    p, _, _, keep = Dataset._params(dev_ptr=0, **kw)
+    rc = lib().idx2g_validate_params(ctypes.byref(p))
+    del keep
+    return rc
+
+
+def max_levels(dims, brick, limit=8):
+    """Largest level count the layout accepts for a grid (0 if none)."""
+    best = 0
+    for nl in range(1, limit + 1):
+        if validate_params(dims=dims, dtype="float32", brick=brick, nlevels=nl) == 0:
+            best = nl
+    return best
+
+
+def decode_batch(ctx, datasets, out_dev_ptrs, queries=None):
+    """Decode query i of datasets[i] into device buffer out_dev_ptrs[i] for every i
+    in one pass (idx2g_decode_batch).  queries: list of make_query() keyword
+    dicts, or None for whole volumes."""
+    n = len(datasets)
+    hs = (ctypes.c_void_p * n)(*[d._h.value if isinstance(d._h, ctypes.c_void_p) else d._h for d in datasets])
+    outs = (ctypes.c_void_p * n)(*[int(p) for p in out_dev_ptrs])
+    qs = None
+    if queries is not None:
+        qs = (Query * n)()
+        for i, q in enumerate(queries):
+            qs[i] = make_query(**q)
+    _check(lib().idx2g_decode_batch(ctx._h, n, hs, qs, outs, 1))
 
 
 class Server:

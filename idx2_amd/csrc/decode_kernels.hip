@@ -77,18 +77,23 @@ __device__ inline void dunpack(uint32_t w, int org[3], int dims[3], int* skip, i
   *ci = int(w >> 25);
 }
 
-__global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __restrict__ ap) {
-  const DecArgs& a = *ap;
+// Bricks of several (dataset, level) jobs per launch: table[j] describes job
+// j, whose bricks are work items start[j] .. start[j + 1] - 1.
+__global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __restrict__ table,
+                                                           const int* __restrict__ start, int njobs) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ int s_klo, s_khi;
-  const BrickConst& k = a.k;
-  const int npl = k.dtype ? 64 : 32;
-  DecLds S = dcarve(lds_raw, npl);
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int prec3[4] = {63, 62, 61, 60};
-
-  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
-    double* F = a.scratch + (size_t)blockIdx.x * k.nvox;
+  const int total = start[njobs];
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const DecArgs& a = table[j];
+    const BrickConst& k = a.k;
+    const int npl = k.dtype ? 64 : 32;
+    DecLds S = dcarve(lds_raw, npl);
+    const int bi = g - start[j];
+    double* F = a.scratch + (size_t)blockIdx.x * a.wstride;
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
     const u64 lin = a.lin[bi];
     const int dense = a.dense[bi];
@@ -265,16 +270,25 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
 
 // Stream walks of full 32^3 bricks (dec32.hpp): one 256-thread workgroup per
 // brick, plane words and counts to the batch arrays a.xs / a.nbps.
-__global__ void __launch_bounds__(b32::kWalkThreads) k_walk_b32(const DecArgs* __restrict__ ap) {
-  const DecArgs& a = *ap;
+__global__ void __launch_bounds__(b32::kWalkThreads) k_walk_b32(const DecArgs* __restrict__ table,
+                                                               const int* __restrict__ start, int njobs) {
   __shared__ b32::WalkLds W;
   __shared__ uint32_t s_blocks[8 * 128];
-  for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
-    const int s = i >> 7, b = i & 127;
-    s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
-  }
-  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
-    b32::walk_wg(a, W, bi, s_blocks);
+  const int total = start[njobs];
+  int cj = -1;
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const DecArgs& a = table[j];
+    if (j != cj) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
+        const int s = i >> 7, b = i & 127;
+        s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
+      }
+      cj = j;
+      __syncthreads();
+    }
+    b32::walk_wg(a, W, g - start[j], s_blocks);
     __syncthreads();
   }
 }
@@ -309,26 +323,36 @@ __device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, con
 // parked in the workgroup's scratch area, so that only the z-even half of the
 // register columns is live while the z-even subbands are decoded.
 template <int DT>
-__global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restrict__ ap) {
-  const DecArgs& a = *ap;
+__global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restrict__ table,
+                                                         const int* __restrict__ start, int njobs) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ uint32_t s_blocks[8 * 128];
   __shared__ double w8[8];
   b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
   b32::DecLds32& D = *reinterpret_cast<b32::DecLds32*>(&S.u);
-  const BrickConst& k = a.k;
-  for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
-    const int s = i >> 7, b = i & 127;
-    s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
-  }
-  if (threadIdx.x < 8) {
-    const int s = threadIdx.x;
-    const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
-    w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.winv[s];
-  }
   constexpr int npl = DT ? 64 : 32;
-  double* Z1 = a.scratch + (size_t)blockIdx.x * k.nvox;
-  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+  const int total = start[njobs];
+  int cj = -1;
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const DecArgs& a = table[j];
+    const BrickConst& k = a.k;
+    if (j != cj) {  // block tables and inverse subband weights of the job
+      __syncthreads();
+      for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
+        const int s = i >> 7, b = i & 127;
+        s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
+      }
+      if (threadIdx.x < 8) {
+        const int s = threadIdx.x;
+        const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
+        w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.winv[s];
+      }
+      cj = j;
+      __syncthreads();
+    }
+    const int bi = g - start[j];
+    double* Z1 = a.scratch + (size_t)blockIdx.x * a.wstride;
     const b32::Pos p = b32::pos();
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
     const int dense = a.dense[bi];

@@ -268,6 +268,14 @@ __device__ __forceinline__ double wave_max(double v) {
   }
   return v;
 }
+__device__ __forceinline__ int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // --------------------------------------------------------------------------

@@ -299,17 +299,23 @@ __device__ inline void store_minmax(const EncArgs& a, EncShared& sh, int dense, 
 
 // Any brick (partial bricks at the volume's far faces, any brick size up to
 // 32): the extended brick lives in a global scratch volume per workgroup.
-__global__ void __launch_bounds__(kThreads) k_encode_level(const EncArgs* __restrict__ ap) {
-  const EncArgs& a = *ap;
+// Bricks of several (dataset, level) jobs in one launch: table[j] describes
+// job j, whose bricks are work items start[j] .. start[j + 1] - 1.
+__global__ void __launch_bounds__(kThreads) k_encode_level(const EncArgs* __restrict__ table,
+                                                           const int* __restrict__ start, int njobs) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ EncShared sh;
-  const BrickConst& k = a.k;
   const int tid = threadIdx.x;
-  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+  const int total = start[njobs];
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const EncArgs& a = table[j];
+    const BrickConst& k = a.k;
+    const int bi = g - start[j];
     const int dense = a.list ? a.list[bi] : a.b0 + bi;
     int b3[3], from[3], ext[3];
     brick_origin(a, dense, b3, from, ext);
-    double* F = a.scratch + (size_t)blockIdx.x * k.nvox;
+    double* F = a.scratch + (size_t)blockIdx.x * a.wstride;
     double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
     for (int i = tid; i < k.nvox; i += blockDim.x) {
       int x = i % k.N[0], y = (i / k.N[0]) % k.N[1], z = i / (k.N[0] * k.N[1]);
@@ -346,27 +352,37 @@ __global__ void __launch_bounds__(kThreads) k_encode_level(const EncArgs* __rest
 // the transformed brick is handed to the coding stages through the
 // workgroup's scratch volume.
 template <int DT>
-__global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restrict__ ap) {
-  const EncArgs& a = *ap;
+__global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restrict__ table,
+                                                         const int* __restrict__ start, int njobs) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ EncShared sh;
   __shared__ double w8[8];
   __shared__ uint32_t s_blocks[8 * 128];
   __shared__ b32::ArenaCursor cur;
   b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
-  for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
-    const int s = i >> 7, b = i & 127;
-    s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
-  }
   if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0};
-  const BrickConst& k = a.k;
-  if (threadIdx.x < 8) {
-    const int s = threadIdx.x;
-    const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
-    w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.wfwd[s];
-  }
-  const unsigned plane = (unsigned)a.gdims[1] * (unsigned)a.gdims[0];
-  for (int bi = blockIdx.x; bi < a.nb; bi += gridDim.x) {
+  const int total = start[njobs];
+  int cj = -1;
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const EncArgs& a = table[j];
+    const BrickConst& k = a.k;
+    if (j != cj) {  // block tables and subband weights of the job
+      __syncthreads();
+      for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
+        const int s = i >> 7, b = i & 127;
+        s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
+      }
+      if (threadIdx.x < 8) {
+        const int s = threadIdx.x;
+        const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
+        w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.wfwd[s];
+      }
+      cj = j;
+      __syncthreads();
+    }
+    const unsigned plane = (unsigned)a.gdims[1] * (unsigned)a.gdims[0];
+    const int bi = g - start[j];
     const int dense = a.list ? a.list[bi] : a.b0 + bi;
     int b3[3], from[3], ext[3];
     brick_origin(a, dense, b3, from, ext);
@@ -406,7 +422,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
     }
     b32::weigh(c, S, p, w8);
     b32::code_brick<DT>(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
-                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * k.nvox), s_blocks, cur);
+                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride), s_blocks, cur);
     __syncthreads();
   }
 }
@@ -423,39 +439,65 @@ __device__ inline int varbyte_len(u64 v) {
   return n;
 }
 
-// Key range of a level from the per-brick-subband headers.
-__global__ void k_key_range(const SbHdr* hdrs, int n, int* out) {
+// Key range of each job's level from its per-brick-subband headers; start[]
+// counts workgroups per job.
+__global__ void k_key_range(const AsmJob* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const AsmJob& J = jobs[j];
+  const int n = J.nb * J.nsb;
+  const int nwg = start[j + 1] - start[j];
   int lo = 0x7fffffff, hi = -0x7fffffff;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    if (hdrs[i].nkeys > 0) {
-      lo = min(lo, hdrs[i].klo);
-      hi = max(hi, hdrs[i].klo + hdrs[i].nkeys - 1);
+  for (int i = (blockIdx.x - start[j]) * blockDim.x + threadIdx.x; i < n; i += nwg * blockDim.x) {
+    if (J.hdrs[i].nkeys > 0) {
+      lo = min(lo, J.hdrs[i].klo);
+      hi = max(hi, J.hdrs[i].klo + J.hdrs[i].nkeys - 1);
     }
   }
-  atomicMin(&out[0], lo);
-  atomicMax(&out[1], hi);
-}
-
-// dense index table: idx[(sb*nk + key-klo)*span + lin] = entry + 1
-__global__ void k_fill_index(const SbHdr* hdrs, const u64* lin, int nb, int nsb, int klo, int nk, u64 span,
-                             uint32_t* idx) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nb * nsb) return;
-  const int dense = i / nsb, sb = i % nsb;
-  const SbHdr h = hdrs[i];
-  for (int j = 0; j < h.nkeys; ++j) {
-    const int kk = h.klo + j - klo;
-    idx[((size_t)sb * nk + kk) * span + lin[dense]] = (uint32_t)(h.ebase + j + 1);
+  // one atomic pair per workgroup
+  lo = wave_min_i(lo);
+  hi = wave_max_i(hi);
+  __shared__ int s_lo[16], s_hi[16];
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane_id() == 0) {
+    s_lo[wave] = lo;
+    s_hi[wave] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < nw; ++w) {
+      lo = min(lo, s_lo[w]);
+      hi = max(hi, s_hi[w]);
+    }
+    if (lo != 0x7fffffff) atomicMin(&J.kr[0], lo);
+    if (hi != -0x7fffffff) atomicMax(&J.kr[1], hi);
   }
 }
 
-// One workgroup per (subband, key, chunk): sizes of the chunk.
-__global__ void __launch_bounds__(256) k_chunk_size(const uint32_t* idx, const SbEntry* entries, int nk, u64 span,
-                                                    int bpc, int nchunks, ChunkRec* recs, int klo) {
-  const int r = blockIdx.x;
+// dense index table: idx[(sb*nk + key-klo)*span + lin] = entry + 1; start[]
+// counts workgroups (256 headers each) per job.
+__global__ void k_fill_index(const AsmJob* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const AsmJob& J = jobs[j];
+  const int i = (blockIdx.x - start[j]) * blockDim.x + threadIdx.x;
+  if (i >= J.nb * J.nsb) return;
+  const int dense = i / J.nsb, sb = i % J.nsb;
+  const SbHdr h = J.hdrs[i];
+  for (int t = 0; t < h.nkeys; ++t) {
+    const int kk = h.klo + t - J.klo;
+    J.idx[((size_t)sb * J.nk + kk) * J.span + J.lin[dense]] = (uint32_t)(h.ebase + t + 1);
+  }
+}
+
+// One workgroup per (job, subband, key, chunk): sizes of the chunk.
+__global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
+                                                    int njobs, const SbEntry* __restrict__ entries) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const AsmJob& J = jobs[j];
+  const int r = blockIdx.x - start[j];
+  const int nchunks = J.nchunks, nk = J.nk;
   const int chunk = r % nchunks, kk = (r / nchunks) % nk, sb = r / (nchunks * nk);
-  const u64 b0 = (u64)chunk * bpc, b1 = min(b0 + (u64)bpc, span);
-  const uint32_t* row = idx + ((size_t)sb * nk + kk) * span;
+  const u64 b0 = (u64)chunk * J.bpc, b1 = min(b0 + (u64)J.bpc, J.span);
+  const uint32_t* row = J.idx + ((size_t)sb * nk + kk) * J.span;
   unsigned long long cnt = 0, sz = 0, vb = 0;
   u64 first = ~0ull, last = 0;
   for (u64 b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
@@ -484,7 +526,7 @@ __global__ void __launch_bounds__(256) k_chunk_size(const uint32_t* idx, const S
     c.off = 0;
     c.nbricks = (uint32_t)s_cnt;
     c.sb = sb;
-    c.key = klo + kk;
+    c.key = J.klo + kk;
     c.chunk = chunk;
     c.first_brick = s_first;
     c.last_brick = s_last;
@@ -495,8 +537,31 @@ __global__ void __launch_bounds__(256) k_chunk_size(const uint32_t* idx, const S
       c.size = (uint32_t)(varbyte_len(s_cnt) + (delta_bits + 7) / 8 + s_vb + s_sz);
       c.first_fb = entries[row[s_first] - 1].first_block;
     }
-    recs[r] = c;
+    J.recs[r] = c;
   }
+}
+
+// Zero the byte ranges [dst, dst + n) of the segments (one workgroup each).
+__global__ void k_zero(const ScatterSeg* __restrict__ segs) {
+  const ScatterSeg sg = segs[blockIdx.x];
+  uint8_t* d = (uint8_t*)sg.dst;
+  const u64 head = min(sg.n, (u64)((16 - ((uintptr_t)d & 15)) & 15));
+  for (u64 i = threadIdx.x; i < head; i += blockDim.x) d[i] = 0;
+  const u64 nv = (sg.n - head) / 16;
+  uint4* dv = reinterpret_cast<uint4*>(d + head);
+  for (u64 i = threadIdx.x; i < nv; i += blockDim.x) dv[i] = make_uint4(0, 0, 0, 0);
+  for (u64 i = head + nv * 16 + threadIdx.x; i < sg.n; i += blockDim.x) d[i] = 0;
+}
+
+// Copy segments of an uploaded slab to their destinations (one workgroup
+// each; 16-byte aligned offsets and destinations).
+__global__ void k_scatter(const uint8_t* __restrict__ slab, const ScatterSeg* __restrict__ segs) {
+  const ScatterSeg sg = segs[blockIdx.x];
+  const uint8_t* s = slab + sg.off;
+  uint8_t* d = (uint8_t*)sg.dst;
+  const u64 nv = sg.n / 16;
+  for (u64 i = threadIdx.x; i < nv; i += blockDim.x) reinterpret_cast<uint4*>(d)[i] = reinterpret_cast<const uint4*>(s)[i];
+  for (u64 i = nv * 16 + threadIdx.x; i < sg.n; i += blockDim.x) d[i] = s[i];
 }
 
 // Byte copy by one thread: aligned 32-bit destination words, each assembled
@@ -526,10 +591,17 @@ __device__ inline void copy_bytes(uint8_t* d, const uint8_t* s, uint32_t n) {
 // One workgroup per non-empty chunk (in blob order): write the header, the
 // varbyte sizes and the payloads, and record where each brick's stream
 // starts (decode directory: blob offset + 1).
-__global__ void __launch_bounds__(256) k_chunk_write(const uint32_t* idx, const SbEntry* entries, const uint8_t* arena,
-                                                     int nk, int klo, u64 span, int bpc, const ChunkRec* recs,
-                                                     const int* order, uint8_t* blob, u64* dir) {
-  const ChunkRec c = recs[order[blockIdx.x]];
+__global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
+                                                     int njobs, const SbEntry* __restrict__ entries,
+                                                     const uint8_t* __restrict__ arena) {
+  const int jb = job_of(start, njobs, blockIdx.x);
+  const AsmJob& J = jobs[jb];
+  const int nk = J.nk, klo = J.klo, bpc = J.bpc;
+  const u64 span = J.span;
+  const uint32_t* idx = J.idx;
+  uint8_t* blob = J.blob;
+  u64* dir = J.dir;
+  const ChunkRec c = J.recs[J.order[blockIdx.x - start[jb]]];
   const size_t rowi = ((size_t)c.sb * nk + (c.key - klo)) * span;
   const uint32_t* row = idx + rowi;
   u64* drow = dir + rowi;

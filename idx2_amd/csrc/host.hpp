@@ -43,6 +43,10 @@ struct idx2g_ctx {
   uint8_t* h_args = nullptr;
   uint8_t* d_args = nullptr;
   int arg_slot = 0;
+  // pinned staging of batched uploads (ev[6] marks the last transfer out of it)
+  uint8_t* h_stage = nullptr;
+  size_t stage_cap = 0;
+  bool stage_pending = false;
 };
 
 struct idx2g_dataset {

@@ -48,7 +48,8 @@ struct EncArgs {
   const int* xyz;        // [nb_level][3]
   const int* list;       // dense ids to process, or null for [b0, b0 + nb)
   int nb, b0;
-  double* scratch;       // [gridDim.x][nvox] (generic kernel: work volume; b32: transformed brick)
+  double* scratch;       // [gridDim.x][wstride] (generic kernel: work volume; b32: parked z-odd half)
+  int wstride;           // doubles per workgroup slot of scratch
   // next level's grid (non-coarsest): subband-0 interiors at brick * sbne
   double* parents;
   int pdims[3];
@@ -67,6 +68,41 @@ struct EncArgs {
   SbEntry* entries;
   SbHdr* hdrs;          // [dense][nsb]
   double* minmax;       // [dense][2] (level 0)
+};
+
+// Multi-job launches: work item g of a launch belongs to job j with
+// start[j] <= g < start[j + 1] (start[njobs] = total items).
+__device__ __forceinline__ int job_of(const int* start, int njobs, int g) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Chunk assembly of one (dataset, level): key range, dense stream index,
+// chunk sizes and chunk bytes (WriteChunk, Source/Core/idx2Write.cpp:207-290).
+struct ChunkRec;
+struct AsmJob {
+  const SbHdr* hdrs;         // [nb][nsb]
+  const u64* lin;            // [nb] linear brick ids
+  uint32_t* idx;             // [nsb][nk][span] entry + 1
+  ChunkRec* recs;            // [nsb][nk][nchunks]
+  const int* order;          // non-empty chunks in blob order
+  uint8_t* blob;
+  u64* dir;
+  int* kr;                   // key range out: [lo, hi]
+  u64 span;
+  int nb, nsb, klo, nk, bpc, nchunks;
+};
+
+// Device copies of small host arrays: bytes [off, off + n) of the uploaded
+// slab go to dst.
+struct ScatterSeg {
+  u64 off, n;
+  void* dst;
 };
 
 // per (subband, key, chunk) assembled chunk
@@ -88,7 +124,8 @@ struct DecArgs {
   const u64* lin;        // [n] linear ids
   const int* dense;      // [n] dense ids (exponent rows)
   int nb;
-  double* scratch;       // per workgroup work volume (generic kernel)
+  double* scratch;       // per workgroup slot: work volume (generic kernel), parked z-odd half (b32)
+  int wstride;           // doubles per workgroup slot
   u64* xs;               // b32: plane words of the batch [brick][subband][128 blocks][npl]
   uint8_t* nbps;         // b32: planes decoded per block [brick][subband][128]
   const double* pgrid;   // level+1 decoded data (dense f64; subband-0 interiors of this level)

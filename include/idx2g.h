@@ -108,6 +108,22 @@ const char* idx2g_last_error_message(void);
 int idx2g_encode(idx2g_ctx* ctx, const idx2g_encode_params* p, const void* input,
                  int input_is_device, idx2g_dataset** out);
 
+/* Validate encode parameters without touching the GPU: the checks of
+ * idx2::Finalize (Source/Core/idx2Common.cpp:358-754, e.g. TooManyLevels when
+ * a level has too few bricks for the transform order) as idx2g_encode runs
+ * them.  Returns 0 when idx2g_encode would accept the parameters. */
+int idx2g_validate_params(const idx2g_encode_params* p);
+
+/* Batched encode: no counterpart in the reference (idx2App encodes one field
+ * per run, Applications/idx2App.cpp:114-168).  n fields of any shapes and
+ * dtypes in one pass: each level's bricks of all fields run in the same
+ * persistent brick-kernel launches and chunk assembly runs once for the
+ * batch.  out[i] is byte-identical to idx2g_encode of field i alone.
+ * inputs[i] is field i's grid in host (inputs_are_device = 0) or device
+ * memory (1).  On error no dataset is returned. */
+int idx2g_encode_batch(idx2g_ctx* ctx, int n, const idx2g_encode_params* params, const void* const* inputs,
+                       int inputs_are_device, idx2g_dataset** out);
+
 /* Write out_dir/name/field.idx2 and out_dir/name/field/L..bin with the file
  * layout of FlushChunks/FlushChunkExponents/WriteMetaFile/PrintStats
  * (Source/Core/idx2Write.cpp:30-413, idx2Common.cpp:149-179). */
@@ -135,6 +151,13 @@ int idx2g_output_dims(const idx2g_dataset* ds, const idx2g_query* q, int64_t dim
  * buffer is zeroed first, so samples outside every brick read 0. */
 int idx2g_decode(idx2g_ctx* ctx, idx2g_dataset* ds, const idx2g_query* q, void* out,
                  int out_is_device);
+
+/* Batched decode: no counterpart in the reference (one query per Decode
+ * call).  Query i (queries NULL: whole volumes) of dataset ds[i] into out[i];
+ * each level's bricks of all queries run in shared launches.  Every result
+ * is bit-identical to idx2g_decode of the same query alone. */
+int idx2g_decode_batch(idx2g_ctx* ctx, int n, idx2g_dataset* const* ds, const idx2g_query* queries,
+                       void* const* out, int out_is_device);
 
 /* Device timings of the last encode/decode call on ctx. */
 int idx2g_last_stats(const idx2g_ctx* ctx, idx2g_stats* s);
