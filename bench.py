@@ -173,6 +173,79 @@ def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30):
             "peak_concurrent": info["peak_running"], "peak_admitted_bytes": info["peak_bytes"]}
 
 
+BATCH_DIMS = (64, 96, 128, 192, 256, 384, 512)
+
+
+def batched_fields(torch, ctx, seed, n_fields=64):
+    """The 64-field mix: dims per axis from BATCH_DIMS, 2D and 3D 50/50, float32
+    and float64 50/50, 2-4 levels clamped to each field's maximum, tolerance
+    1e-4, field f generated with seed + f."""
+    from idx2_amd import api
+    rng = np.random.default_rng(seed + 3000)
+    fields, bufs = [], []
+    for f in range(n_fields):
+        is2d = bool(rng.random() < 0.5)
+        dims = [int(rng.choice(BATCH_DIMS)) for _ in range(3)]
+        if is2d:
+            dims[2] = 1
+        dt = "float64" if rng.random() < 0.5 else "float32"
+        brick = (32, 32, 1) if is2d else (32, 32, 32)
+        nl = max(1, min(int(rng.integers(2, 5)), api.max_levels(dims, brick)))
+        buf = torch.empty(dims[0] * dims[1] * dims[2], dtype=torch.float64 if dt == "float64" else torch.float32,
+                          device="cuda")
+        ctx.synth_field(buf.data_ptr(), dims, dt, seed + f)
+        bufs.append(buf)
+        fields.append(dict(dev_ptr=buf.data_ptr(), dims=tuple(dims), dtype=dt, brick=brick, nlevels=nl,
+                           tolerance=1e-4, name=f"F{f}", field="f"))
+    return fields, bufs
+
+
+def batched_bench(ctx, torch, seed, reps=3):
+    """Encode + full decode of the 64-field mix: one call per field (the plain
+    path) against one idx2g_encode_batch + one idx2g_decode_batch."""
+    from idx2_amd import api
+    fields, bufs = batched_fields(torch, ctx, seed)
+    outs = [torch.empty_like(b) for b in bufs]
+    raw = sum(b.numel() * b.element_size() for b in bufs)
+    res = {}
+    for mode in ("per_field", "batched"):
+        enc, dec = [], []
+        for rep in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if mode == "per_field":
+                dss = [api.Dataset.encode(ctx, **f) for f in fields]
+            else:
+                dss = api.Dataset.encode_batch(ctx, fields)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if mode == "per_field":
+                for ds, o in zip(dss, outs):
+                    ds.decode(out_dev_ptr=o.data_ptr())
+            else:
+                api.decode_batch(ctx, dss, [o.data_ptr() for o in outs])
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            if rep:  # first round warms the device memory cache
+                enc.append(t1 - t0)
+                dec.append(t2 - t1)
+            if mode == "batched" and rep == reps:
+                chunk = sum(ds.info()["chunk_bytes"] for ds in dss)
+            for ds in dss:
+                ds.free()
+        e, d = float(np.mean(enc)), float(np.mean(dec))
+        res[mode] = {"encode_ms": round(e * 1e3, 3), "decode_ms": round(d * 1e3, 3),
+                     "gbps": round(raw / (e + d) / 1e9, 3)}
+    res["speedup"] = round(res["batched"]["gbps"] / res["per_field"]["gbps"], 3)
+    ok = all(bool(torch.equal(o, b)) or float((o.double() - b.double()).abs().max()) < 1.0 for o, b in zip(outs, bufs))
+    res.update({"fields": len(fields), "raw_bytes": int(raw), "chunk_bytes": int(chunk),
+                "fields_2d": sum(1 for f in fields if f["dims"][2] == 1),
+                "fields_f64": sum(1 for f in fields if f["dtype"] == "float64"),
+                "levels": [f["nlevels"] for f in fields], "decoded": ok})
+    del bufs, outs
+    return res
+
+
 PROGRESSIVE = dict(dims=(2048, 2048, 1024), dtype="float32", nlevels=3, tolerance=1e-6, brick=(32, 32, 32))
 
 
@@ -237,6 +310,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300, help="serving-loop requests (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="serving-loop arrival rate (requests/s)")
     ap.add_argument("--progressive", action="store_true", help="also run the progressive decode sweep")
+    ap.add_argument("--batched", action="store_true", help="also run the 64-field batched-vs-per-field comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration under rocprofv3) and exit")
     args = ap.parse_args()
@@ -339,6 +413,8 @@ def main():
         res["serving"] = serve_bench(ctx, torch, args.serve_requests, args.serve_rate, args.seed)
     if args.progressive:
         res["progressive"] = progressive_bench(ctx, torch, args.seed)
+    if args.batched:
+        res["batched"] = batched_bench(ctx, torch, args.seed)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(ctx, torch, args.config, args.seed)
     print(json.dumps(res))
