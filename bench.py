@@ -40,6 +40,11 @@ CPU_SAMPLE = {
 }
 
 
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of the dominant kernel from its committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, each calibrated against a plain device copy); None if absent."""
@@ -90,7 +95,7 @@ def cpu_baseline(ctx, torch, cfg_name, seed):
 SERVE_SETS = [((512, 512, 512), "float64"), ((1024, 1024, 1024), "float32"), ((2048, 2048, 512), "float32")]
 
 
-def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30):
+def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=48 << 30):
     import threading
     from idx2_amd import api
     sets = []
@@ -139,26 +144,38 @@ def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30):
             read_bytes += st["chunk_bytes"]
             del buf
 
-    th = threading.Thread(target=waiter)
+    th = threading.Thread(target=waiter, daemon=True)
     th.start()
     t0 = time.perf_counter()
     t_next = t0
-    for ds, dt, q, gap in reqs:
-        t_next += gap
-        now = time.perf_counter()
-        if t_next > now:
-            time.sleep(t_next - now)
-        dims, nbytes = ds.output_dims(**q)
-        buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
-                          dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
-        rid = server.submit(ds, buf.data_ptr(), **q)
+    stalls = 0
+    try:
+        for ds, dt, q, gap in reqs:
+            t_next += gap
+            now = time.perf_counter()
+            if t_next > now:
+                time.sleep(t_next - now)
+            dims, nbytes = ds.output_dims(**q)
+            # the client holds at most max_out bytes of output buffers (an
+            # overloaded server makes it wait here; counted as stalls)
+            while True:
+                with lock:
+                    held = sum(b.numel() * b.element_size() for _, b in pending)
+                if held + nbytes <= max_out or held == 0:
+                    break
+                stalls += 1
+                time.sleep(0.001)
+            buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
+                              dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+            rid = server.submit(ds, buf.data_ptr(), **q)
+            with lock:
+                pending.append((rid, buf))
+                lock.notify()
+    finally:
         with lock:
-            pending.append((rid, buf))
+            done_flag[0] = True
             lock.notify()
-    with lock:
-        done_flag[0] = True
-        lock.notify()
-    th.join()
+        th.join()
     wall = time.perf_counter() - t0
     info = server.info()
     server.close()
@@ -170,7 +187,8 @@ def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30):
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_queue_ms": round(float(np.mean(qms)), 3), "served_per_s": round(n_requests / wall, 2),
             "output_gbps": round(out_bytes / wall / 1e9, 3), "stream_bytes_read": int(read_bytes),
-            "peak_concurrent": info["peak_running"], "peak_admitted_bytes": info["peak_bytes"]}
+            "peak_concurrent": info["peak_running"], "peak_admitted_bytes": info["peak_bytes"],
+            "client_stalls": stalls}
 
 
 BATCH_DIMS = (64, 96, 128, 192, 256, 384, 512)
@@ -299,6 +317,63 @@ def progressive_bench(ctx, torch, seed, n_roi=32):
     return {"encode_ms": round(enc_ms, 2), "queries": len(rows), "sweep": summary}
 
 
+def plain_bench(ctx, torch, cfg, steps, warmup, seed):
+    """Encode + full decode of one grid resident in HBM, `steps` timed rounds."""
+    from idx2_amd import api
+    nx, ny, nz = cfg["dims"]
+    dt = cfg["dtype"]
+    tdt = torch.float64 if dt == "float64" else torch.float32
+    inp = torch.empty(nx * ny * nz, dtype=tdt, device="cuda")
+    ctx.synth_field(inp.data_ptr(), cfg["dims"], dt, seed)
+    out = torch.empty_like(inp)
+    raw = inp.numel() * inp.element_size()
+    kw = dict(dims=cfg["dims"], dtype=dt, brick=cfg["brick"], nlevels=cfg["nlevels"], tolerance=cfg["tolerance"])
+
+    def step():
+        ds = api.Dataset.encode(ctx, dev_ptr=inp.data_ptr(), name="bench", field="f", **kw)
+        es = ctx.stats()
+        ds.decode(out_dev_ptr=out.data_ptr())
+        dsx = ctx.stats()
+        chunk_bytes = ds.info()["chunk_bytes"]
+        ds.free()
+        return es, dsx, chunk_bytes
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    enc, dec = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        es, dsx, chunk_bytes = step()
+        enc.append(es)
+        dec.append(dsx)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # error check of the last round trip (size-independent property: |x - x'| small)
+    err = float((out - inp).abs().max().item())
+    del inp, out
+    enc_ms = np.array([e["total_ms"] for e in enc])
+    dec_ms = np.array([d["total_ms"] for d in dec])
+    # dominant kernel: the level-0 brick launches of encode or decode, whichever is longer
+    on_chip = tuple(cfg["brick"]) == (32, 32, 32) and nz > 1
+    dom_e = float(np.mean([e["dominant_ms"] for e in enc]))
+    dom_d = float(np.mean([d["dominant_ms"] for d in dec]))
+    if dom_e >= dom_d:
+        name = "idx2g::k_encode_b32 (level 0)" if on_chip else "idx2g::k_encode_level (level 0)"
+        dom_ms, dom_bytes, launches = dom_e, float(np.mean([e["dominant_bytes"] for e in enc])), 1
+    else:
+        name = ("idx2g::k_walk_b32 + idx2g::k_decode_b32 (level 0)" if on_chip
+                else "idx2g::k_decode_level (level 0)")
+        dom_ms, dom_bytes = dom_d, float(np.mean([d["dominant_bytes"] for d in dec]))
+        launches = int(np.mean([d["dominant_launches"] for d in dec]))
+    return {"value": raw * steps / wall / 1e9, "wall": wall, "enc_ms": float(enc_ms.mean()),
+            "dec_ms": float(dec_ms.mean()), "enc_gbs": raw / (enc_ms.mean() / 1e3) / 1e9,
+            "dec_gbs": raw / (dec_ms.mean() / 1e3) / 1e9, "dec_p50": float(np.percentile(dec_ms, 50)),
+            "dec_p99": float(np.percentile(dec_ms, 99)), "chunk_bytes": chunk_bytes, "err": err,
+            "dom_name": name, "dom_ms": dom_ms, "dom_bytes": dom_bytes, "dom_launches": launches,
+            "achieved": dom_bytes / (dom_ms / 1e3) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -307,10 +382,14 @@ def main():
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=300, help="serving-loop requests (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=200.0, help="serving-loop arrival rate (requests/s)")
-    ap.add_argument("--progressive", action="store_true", help="also run the progressive decode sweep")
-    ap.add_argument("--batched", action="store_true", help="also run the 64-field batched-vs-per-field comparison")
+    ap.add_argument("--serve-requests", type=int, default=10000, help="serving-loop requests (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="serving-loop arrival rate (requests/s)")
+    ap.add_argument("--no-progressive", dest="progressive", action="store_false",
+                    help="skip the progressive decode sweep")
+    ap.add_argument("--no-batched", dest="batched", action="store_false",
+                    help="skip the 64-field batched-vs-per-field comparison")
+    ap.add_argument("--no-slice", dest="slice2d", action="store_false",
+                    help="skip the 2D float64 8192^2 slice")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration under rocprofv3) and exit")
     args = ap.parse_args()
@@ -329,93 +408,57 @@ def main():
         return
     from idx2_amd import api
 
-    cfg = CONFIGS[args.config]
-    nx, ny, nz = cfg["dims"]
-    dt = cfg["dtype"]
-    tdt = torch.float64 if dt == "float64" else torch.float32
     ctx = api.Context(0)
-    inp = torch.empty(nx * ny * nz, dtype=tdt, device="cuda")
-    ctx.synth_field(inp.data_ptr(), cfg["dims"], dt, args.seed)
-    out = torch.empty_like(inp)
-    raw = inp.numel() * inp.element_size()
-    kw = dict(dims=cfg["dims"], dtype=dt, brick=cfg["brick"], nlevels=cfg["nlevels"], tolerance=cfg["tolerance"])
-
-    def step():
-        ds = api.Dataset.encode(ctx, dev_ptr=inp.data_ptr(), name="bench", field="f", **kw)
-        es = ctx.stats()
-        ds.decode(out_dev_ptr=out.data_ptr())
-        dsx = ctx.stats()
-        chunk_bytes = ds.info()["chunk_bytes"]
-        ds.free()
-        return es, dsx, chunk_bytes
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    enc, dec = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        es, dsx, chunk_bytes = step()
-        enc.append(es)
-        dec.append(dsx)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    # error check of the last round trip (size-independent property: |x - x'| small)
-    err = float((out - inp).abs().max().item())
-    enc_ms = np.array([e["total_ms"] for e in enc])
-    dec_ms = np.array([d["total_ms"] for d in dec])
-    enc_gbs = raw / (enc_ms.mean() / 1e3) / 1e9
-    dec_gbs = raw / (dec_ms.mean() / 1e3) / 1e9
-    value = raw * args.steps / wall / 1e9
-    # dominant kernel: the level-0 brick kernel of encode or decode, whichever is longer
-    dom_e = np.mean([e["dominant_ms"] for e in enc])
-    dom_d = np.mean([d["dominant_ms"] for d in dec])
-    if dom_e >= dom_d:
-        dom_name, dom_ms = "idx2g::k_encode_b32 (level 0)", dom_e
-        dom_bytes = np.mean([e["dominant_bytes"] for e in enc])
-        dom_launches = 1
-    else:
-        dom_name, dom_ms = "idx2g::k_walk_b32 + idx2g::k_decode_b32 (level 0)", dom_d
-        dom_bytes = np.mean([d["dominant_bytes"] for d in dec])
-        dom_launches = int(np.mean([d["dominant_launches"] for d in dec]))
-    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    traffic = measured_traffic(dom_name) if args.config == "c2" else None
+    cfg = CONFIGS[args.config]
+    log(f"plain path: {cfg['workload']}")
+    m = plain_bench(ctx, torch, cfg, args.steps, args.warmup, args.seed)
+    traffic = measured_traffic(m["dom_name"]) if args.config == "c2" else None
     res = {
         "metric": METRIC,
-        "value": round(value, 3),
+        "value": round(m["value"], 3),
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(m["wall"] / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Gaussian+sine+noise field generated on the GPU)",
-        "config": {"workload": cfg["workload"], "dims": list(cfg["dims"]), "field_dtype": dt,
+        "config": {"workload": cfg["workload"], "dims": list(cfg["dims"]), "field_dtype": cfg["dtype"],
                    "levels": cfg["nlevels"], "tolerance": cfg["tolerance"], "brick": list(cfg["brick"])},
-        "encode_gbps": round(enc_gbs, 3),
-        "decode_gbps": round(dec_gbs, 3),
-        "encode_ms": round(float(enc_ms.mean()), 3),
-        "decode_ms": round(float(dec_ms.mean()), 3),
-        "decode_p50_ms": round(float(np.percentile(dec_ms, 50)), 3),
-        "decode_p99_ms": round(float(np.percentile(dec_ms, 99)), 3),
-        "compressed_bytes": int(chunk_bytes),
-        "max_abs_err": err,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom_name,
-                     "kernel_ms": round(float(dom_ms), 3), "launches": dom_launches,
-                     "algorithmic_bytes": int(dom_bytes)},
+        "encode_gbps": round(m["enc_gbs"], 3),
+        "decode_gbps": round(m["dec_gbs"], 3),
+        "encode_ms": round(m["enc_ms"], 3),
+        "decode_ms": round(m["dec_ms"], 3),
+        "decode_p50_ms": round(m["dec_p50"], 3),
+        "decode_p99_ms": round(m["dec_p99"], 3),
+        "compressed_bytes": int(m["chunk_bytes"]),
+        "max_abs_err": m["err"],
+        "roofline": {"bound": "hbm", "achieved": round(m["achieved"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(m["achieved"] / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": m["dom_name"],
+                     "kernel_ms": round(m["dom_ms"], 3), "launches": m["dom_launches"],
+                     "algorithmic_bytes": int(m["dom_bytes"])},
     }
-    del inp, out
+    if args.slice2d and args.config == "c2":
+        log("2D slice")
+        s2 = plain_bench(ctx, torch, CONFIGS["c2b"], 2, 1, args.seed)
+        res["slice_2d"] = {"workload": CONFIGS["c2b"]["workload"], "gbps": round(s2["value"], 3),
+                           "encode_ms": round(s2["enc_ms"], 3), "decode_ms": round(s2["dec_ms"], 3),
+                           "kernel": s2["dom_name"], "kernel_ms": round(s2["dom_ms"], 3),
+                           "kernel_achieved_gbps": round(s2["achieved"], 2), "max_abs_err": s2["err"]}
     if args.serve_requests > 0:
+        log(f"serving loop: {args.serve_requests} requests at {args.serve_rate}/s")
         res["serving"] = serve_bench(ctx, torch, args.serve_requests, args.serve_rate, args.seed)
     if args.progressive:
+        log("progressive sweep")
         res["progressive"] = progressive_bench(ctx, torch, args.seed)
     if args.batched:
+        log("batched 64-field mix")
         res["batched"] = batched_bench(ctx, torch, args.seed)
     if not args.no_cpu_baseline:
+        log("CPU baseline")
         res["cpu_baseline"] = cpu_baseline(ctx, torch, args.config, args.seed)
     print(json.dumps(res))
 

@@ -352,13 +352,15 @@ __device__ __forceinline__ void zpass(double (&c)[kN], Lds& S, const Pos& p) {
   }
 }
 
-// Lines of the LDS subgrid along axis a with index stride s and D points.
-template <int MODE>
-__device__ __forceinline__ void sub_pass(double* sub, int a, int s, int D, const Pos& p) {
-  const int st[3] = {1, 17, 289};
-  const int sa = st[a] * s, s1 = st[a == 0 ? 1 : 0] * s, s2 = st[a == 2 ? 1 : 2] * s;
-  const int per = 64 / D, slot = p.lane / D, i = p.lane - slot * D;
-  const int nl = D * D;
+// Lines of the LDS subgrid along axis A with index stride SS and D points
+// (all compile-time, so the line/point split is shifts and multiplies).
+template <int MODE, int A, int SS, int D>
+__device__ __forceinline__ void sub_pass(double* sub, const Pos& p) {
+  constexpr int sa = (A == 0 ? 1 : A == 1 ? 17 : 289) * SS;
+  constexpr int s1 = (A == 0 ? 17 : 1) * SS;
+  constexpr int s2 = (A == 2 ? 17 : 289) * SS;
+  constexpr int per = 64 / D, nl = D * D;
+  const int slot = p.lane / D, i = p.lane - slot * D;
   for (int j0 = p.w * per; j0 < nl; j0 += (kTh / 64) * per) {
     const int j = j0 + slot;
     const bool act = slot < per && j < nl;
@@ -366,6 +368,29 @@ __device__ __forceinline__ void sub_pass(double* sub, int a, int s, int D, const
     double v = act ? *q : 0.0;
     v = lift_lanes<MODE>(v, i, D);
     if (act) *q = v;
+  }
+}
+
+// One level of the subgrid transform (level L >= 1: stride 2^(L-1), D points
+// per line): X, Y, Z forward or Z, Y, X inverse.
+template <int MODE, int L>
+__device__ __forceinline__ void sub_level(double* sub, const Pos& p) {
+  constexpr int SS = 1 << (L - 1);
+  constexpr int D = L == 1 ? 17 : L == 2 ? 9 : L == 3 ? 5 : 3;
+  if (MODE == kFwd) {
+    sub_pass<MODE, 0, SS, D>(sub, p);
+    __syncthreads();
+    sub_pass<MODE, 1, SS, D>(sub, p);
+    __syncthreads();
+    sub_pass<MODE, 2, SS, D>(sub, p);
+    __syncthreads();
+  } else {
+    sub_pass<MODE, 2, SS, D>(sub, p);
+    __syncthreads();
+    sub_pass<MODE, 1, SS, D>(sub, p);
+    __syncthreads();
+    sub_pass<MODE, 0, SS, D>(sub, p);
+    __syncthreads();
   }
 }
 
@@ -404,21 +429,14 @@ __device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& 
   __syncthreads();
   sub_copy<true>(c, S, p);
   __syncthreads();
-  const int Ds[5] = {33, 17, 9, 5, 3};
-  for (int l = 1; l <= 4; ++l) {
-    const int s = 1 << (l - 1);
-    for (int a = 0; a < 3; ++a) {
-      sub_pass<kFwd>(S.u.sub, a, s, Ds[l], p);
-      __syncthreads();
-    }
-  }
-  for (int l = 4; l >= 1; --l) {
-    const int s = 1 << (l - 1);
-    for (int a = 2; a >= 0; --a) {
-      sub_pass<kInv>(S.u.sub, a, s, Ds[l], p);
-      __syncthreads();
-    }
-  }
+  sub_level<kFwd, 1>(S.u.sub, p);
+  sub_level<kFwd, 2>(S.u.sub, p);
+  sub_level<kFwd, 3>(S.u.sub, p);
+  sub_level<kFwd, 4>(S.u.sub, p);
+  sub_level<kInv, 4>(S.u.sub, p);
+  sub_level<kInv, 3>(S.u.sub, p);
+  sub_level<kInv, 2>(S.u.sub, p);
+  sub_level<kInv, 1>(S.u.sub, p);
   sub_copy<false>(c, S, p);
   __syncthreads();
   zpass<kInv>(c, S, p);

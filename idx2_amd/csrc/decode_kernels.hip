@@ -48,24 +48,24 @@ __device__ __forceinline__ u64 decode_plane(const uint8_t* blob, u64& pos, int& 
 }
 
 struct DecLds {
-  u64* pw;              // [kMaxBlocks][npl]
-  short* emax;          // [kMaxBlocks]
-  unsigned char* nbps;  // [kMaxBlocks] planes decoded
-  unsigned char* nsave; // [kMaxBlocks] N after the last key group
-  unsigned char* ssave; // [kMaxBlocks] stopped flag after the last key group
+  u64* pw;              // [mb][npl]
+  short* emax;          // [mb]
+  unsigned char* nbps;  // [mb] planes decoded
+  unsigned char* nsave; // [mb] N after the last key group
+  unsigned char* ssave; // [mb] stopped flag after the last key group
 };
 
-__host__ __device__ inline size_t dec_lds_bytes(int npl) {
-  return size_t(kMaxBlocks) * npl * 8 + kMaxBlocks * (2 + 3) + 64;
+__host__ __device__ inline size_t dec_lds_bytes(int npl, int mb) {
+  return size_t(mb) * npl * 8 + size_t(mb) * (2 + 3) + 64;
 }
 
-__device__ inline DecLds dcarve(unsigned char* base, int npl) {
+__device__ inline DecLds dcarve(unsigned char* base, int npl, int mb) {
   DecLds s;
   size_t o = 0;
-  s.pw = (u64*)(base + o); o += size_t(kMaxBlocks) * npl * 8;
-  s.emax = (short*)(base + o); o += kMaxBlocks * 2;
-  s.nbps = base + o; o += kMaxBlocks;
-  s.nsave = base + o; o += kMaxBlocks;
+  s.pw = (u64*)(base + o); o += size_t(mb) * npl * 8;
+  s.emax = (short*)(base + o); o += size_t(mb) * 2;
+  s.nbps = base + o; o += mb;
+  s.nsave = base + o; o += mb;
   s.ssave = base + o;
   return s;
 }
@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
     const DecArgs& a = table[j];
     const BrickConst& k = a.k;
     const int npl = k.dtype ? 64 : 32;
-    DecLds S = dcarve(lds_raw, npl);
+    DecLds S = dcarve(lds_raw, npl, a.lds_blocks);
     const int bi = g - start[j];
     double* F = a.scratch + (size_t)blockIdx.x * a.wstride;
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};

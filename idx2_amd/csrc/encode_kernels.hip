@@ -20,35 +20,36 @@ struct EncShared {
   uint32_t total;
 };
 
-// LDS layout for one subband (dynamic shared memory), npl planes per block
+// LDS layout for one subband (dynamic shared memory): npl planes per block,
+// room for mb blocks (the launch's largest subband, at most kMaxBlocks), so
+// launches of small bricks (2D) keep several workgroups per CU.
 struct EncLds {
-  u64* pw;          // [kMaxBlocks][npl] plane words
-  uint32_t* poff;   // [kMaxBlocks][npl] bit offset inside the key stream
-  uint16_t* plen;   // [kMaxBlocks][npl] emitted bits
-  short* emax;      // [kMaxBlocks]
-  unsigned char* np;// [kMaxBlocks]
+  u64* pw;          // [mb][npl] plane words
+  uint32_t* poff;   // [mb][npl] bit offset inside the key stream
+  uint16_t* plen;   // [mb][npl] emitted bits
+  short* emax;      // [mb]
+  unsigned char* np;// [mb]
   uint32_t* kbits;  // [kMaxKeys]
   uint32_t* kboff;  // [kMaxKeys] byte offset of the key stream in the subband
   short* kfirst;    // [kMaxKeys]
   uint32_t* stage;  // [kStageWords]
 };
 
-__host__ __device__ inline size_t enc_lds_bytes(int npl) {
-  return size_t(kMaxBlocks) * npl * (8 + 4 + 2) + kMaxBlocks * 3 + kMaxKeys * (4 + 4 + 2) + 64 +
-         size_t(kStageWords) * 4 + 256;
+__host__ __device__ inline size_t enc_lds_bytes(int npl, int mb) {
+  return size_t(mb) * npl * (8 + 4 + 2) + size_t(mb) * 3 + kMaxKeys * (4 + 4 + 2) + 64 + size_t(kStageWords) * 4 + 256;
 }
 
-__device__ inline EncLds carve(unsigned char* base, int npl) {
+__device__ inline EncLds carve(unsigned char* base, int npl, int mb) {
   EncLds s;
   size_t o = 0;
-  s.pw = (u64*)(base + o); o += size_t(kMaxBlocks) * npl * 8;
+  s.pw = (u64*)(base + o); o += size_t(mb) * npl * 8;
   s.stage = (uint32_t*)(base + o); o += size_t(kStageWords) * 4;
-  s.poff = (uint32_t*)(base + o); o += size_t(kMaxBlocks) * npl * 4;
+  s.poff = (uint32_t*)(base + o); o += size_t(mb) * npl * 4;
   s.kbits = (uint32_t*)(base + o); o += kMaxKeys * 4;
   s.kboff = (uint32_t*)(base + o); o += kMaxKeys * 4;
-  s.plen = (uint16_t*)(base + o); o += size_t(kMaxBlocks) * npl * 2;
+  s.plen = (uint16_t*)(base + o); o += size_t(mb) * npl * 2;
   s.kfirst = (short*)(base + o); o += kMaxKeys * 2;
-  s.emax = (short*)(base + o); o += kMaxBlocks * 2;
+  s.emax = (short*)(base + o); o += size_t(mb) * 2;
   s.np = (unsigned char*)(base + o);
   return s;
 }
@@ -100,7 +101,7 @@ __device__ void code_subbands(const EncArgs& a, const double* F, int dense, unsi
                               EncShared& sh) {
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
-  EncLds S = carve(lds_raw, npl);
+  EncLds S = carve(lds_raw, npl, a.lds_blocks);
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int prec3[4] = {63, 62, 61, 60};
   for (int sb = 0; sb < k.nsb; ++sb) {

@@ -86,7 +86,7 @@ typedef struct {
   double transform_ms;    /* encode: transform+coding kernels; decode: decode kernels */
   double assemble_ms;     /* encode: chunk assembly kernels; decode: output/copy kernels */
   int64_t dominant_launches;
-  double dominant_ms;     /* summed duration of the dominant kernel's launches (level-0 brick kernel) */
+  double dominant_ms;     /* device time of the output level's brick launches (encode: level 0) */
   int64_t dominant_bytes; /* algorithmic bytes of those launches: encode raw input + streams written;
                              decode chunk streams read + output grid written */
   int64_t raw_bytes;      /* encode: input bytes; decode: output grid bytes */
