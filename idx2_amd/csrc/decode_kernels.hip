@@ -93,7 +93,10 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
     const int npl = k.dtype ? 64 : 32;
     DecLds S = dcarve(lds_raw, npl, a.lds_blocks);
     const int bi = g - start[j];
-    double* F = a.scratch + (size_t)blockIdx.x * a.wstride;
+    // the extended brick: in LDS when the launch's bricks fit there (small
+    // bricks, 2D), else in the workgroup's scratch slot
+    double* F = a.lds_brick >= 0 ? reinterpret_cast<double*>(lds_raw + a.lds_brick)
+                                 : a.scratch + (size_t)blockIdx.x * a.wstride;
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
     const u64 lin = a.lin[bi];
     const int dense = a.dense[bi];

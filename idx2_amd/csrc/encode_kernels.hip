@@ -316,7 +316,10 @@ __global__ void __launch_bounds__(kThreads) k_encode_level(const EncArgs* __rest
     const int dense = a.list ? a.list[bi] : a.b0 + bi;
     int b3[3], from[3], ext[3];
     brick_origin(a, dense, b3, from, ext);
-    double* F = a.scratch + (size_t)blockIdx.x * a.wstride;
+    // the extended brick: in LDS when the launch's bricks fit there (small
+    // bricks, 2D), else in the workgroup's scratch slot
+    double* F = a.lds_brick >= 0 ? reinterpret_cast<double*>(lds_raw + a.lds_brick)
+                                 : a.scratch + (size_t)blockIdx.x * a.wstride;
     double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
     for (int i = tid; i < k.nvox; i += blockDim.x) {
       int x = i % k.N[0], y = (i / k.N[0]) % k.N[1], z = i / (k.N[0] * k.N[1]);
@@ -589,6 +592,22 @@ __device__ inline void copy_bytes(uint8_t* d, const uint8_t* s, uint32_t n) {
   for (; i < n; ++i) d[i] = s[i];
 }
 
+// Byte copy by a whole wave: aligned 32-bit destination words, each lane
+// assembling its word from two aligned source words (sources are padded by
+// >= 4 bytes).
+__device__ inline void copy_bytes_wave(uint8_t* d, const uint8_t* s, uint32_t n, int lane) {
+  const uint32_t head = min(n, (4u - (uint32_t)((uintptr_t)d & 3)) & 3u);
+  if ((uint32_t)lane < head) d[lane] = s[lane];
+  const uint32_t nw = (n - head) >> 2;
+  uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
+  const uint8_t* sp = s + head;
+  const uint32_t sh = (uint32_t)((uintptr_t)sp & 3);
+  const uint32_t* sw = reinterpret_cast<const uint32_t*>(sp - sh);
+  for (uint32_t k = lane; k < nw; k += 64) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
+  const uint32_t done = head + nw * 4;
+  if ((uint32_t)lane < n - done) d[done + lane] = s[done + lane];
+}
+
 // One workgroup per non-empty chunk (in blob order): write the header, the
 // varbyte sizes and the payloads, and record where each brick's stream
 // starts (decode directory: blob offset + 1).
@@ -670,16 +689,26 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
     }
     const u64 vboff = s_vboff + s1[tid] - v1;
     const u64 doff = s_doff + s2[tid] - sz;
+    const u64 dst = c.off + payload_start + doff;
     if (sz) {
       uint8_t* vp = out + sizes_off + vboff;
       u64 v = sz;
       int i = 0;
       do { uint8_t byte = v & 0x7f; v >>= 7; vp[i++] = byte | (v ? 0x80 : 0); } while (v);
-      const u64 dst = c.off + payload_start + doff;
       drow[b] = dir_entry(dst, sz);
-        copy_bytes(blob + dst, arena + src, sz);  // one thread per stream
-      }
-      __syncthreads();
+    }
+    // payloads: the wave copies its 64 lanes' streams one after another,
+    // every lane on each stream (coalesced words)
+    for (int j = 0; j < 64; ++j) {
+      const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)sz, j);
+      if (!n) continue;
+      const u64 d = ((u64)(uint32_t)__builtin_amdgcn_readlane((int)(dst >> 32), j) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)dst, j);
+      const u64 sr = ((u64)(uint32_t)__builtin_amdgcn_readlane((int)(src >> 32), j) << 32) |
+                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)src, j);
+      copy_bytes_wave(blob + d, arena + sr, n, lane);
+    }
+    __syncthreads();
     if (tid == 255) { s_vboff += s1[255]; s_doff += s2[255]; }
     __syncthreads();
   }

@@ -51,6 +51,7 @@ struct EncArgs {
   double* scratch;       // [gridDim.x][wstride] (generic kernel: work volume; b32: parked z-odd half)
   int wstride;           // doubles per workgroup slot of scratch
   int lds_blocks;        // generic kernel: blocks per subband its LDS layout holds (launch-wide)
+  int lds_brick;         // generic kernel: byte offset of the extended brick in LDS, -1: in scratch
   // next level's grid (non-coarsest): subband-0 interiors at brick * sbne
   double* parents;
   int pdims[3];
@@ -128,6 +129,7 @@ struct DecArgs {
   double* scratch;       // per workgroup slot: work volume (generic kernel), parked z-odd half (b32)
   int wstride;           // doubles per workgroup slot
   int lds_blocks;        // generic kernel: blocks per subband its LDS layout holds (launch-wide)
+  int lds_brick;         // generic kernel: byte offset of the extended brick in LDS, -1: in scratch
   u64* xs;               // b32: plane words of the batch [brick][subband][128 blocks][npl]
   uint8_t* nbps;         // b32: planes decoded per block [brick][subband][128]
   const double* pgrid;   // level+1 decoded data (dense f64; subband-0 interiors of this level)
