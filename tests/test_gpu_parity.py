@@ -117,6 +117,46 @@ def test_2d_f64_slice(ctx, oracle_lib):
                     [dict(), dict(downsampling=(2, 2, 0)), dict(first=(100, 37, 0), last=(900, 700, 0))])
 
 
+EDGE = [
+    # slivers: partial bricks of 1 and 8 samples, float64, tight tolerance
+    ((33, 40, 65), "float64", 1, 1e-6, (32, 32, 32)),
+    # tolerance = machine epsilon of float32 (idx2App's default,
+    # Applications/idx2App.cpp:151): every plane down to the last
+    ((96, 64, 64), "float32", 1, 2.0 ** -23, (32, 32, 32)),
+    # 2D, coarse tolerance
+    ((64, 64, 1), "float32", 1, 1e-2, (32, 32, 1)),
+    # partial bricks on two levels
+    ((130, 66, 34), "float32", 2, 1e-5, (32, 32, 32)),
+    # 16^3 bricks (generic kernels, bricks kept in LDS)
+    ((48, 48, 48), "float64", 2, 1e-4, (16, 16, 16)),
+]
+
+
+@pytest.mark.parametrize("dims,dtype,nlevels,tol,brick", EDGE, ids=[f"{d[0]}x{d[1]}x{d[2]}_{t}_{n}l" for d, t, n, _, _ in EDGE])
+def test_edge_cases_match_oracle(ctx, oracle_lib, dims, dtype, nlevels, tol, brick):
+    print(f"edge case {dims} {dtype} {nlevels} levels tol {tol} brick {brick}", flush=True)
+    last = tuple(d - 1 for d in dims)
+    corner = tuple(max(0, d - 3) for d in dims)
+    flat = dims[2] == 1
+    queries = [dict(), dict(first=corner, last=last), dict(first=last, last=last),
+               dict(downsampling=(1, 2, 0)), dict(tolerance=1e-1),
+               dict(first=(1, 2, 0 if flat else 3), last=(dims[0] - 2, dims[1] - 1, 0 if flat else dims[2] - 2),
+                    downsampling=(0, 1, 0 if flat else 1), tolerance=1e-3)]
+    _roundtrip_case(ctx, oracle_lib, dims, dtype, nlevels, tol, brick, queries, seed=29)
+
+
+def test_empty_query_is_an_error(ctx):
+    """A query box that is empty after clamping to the volume is rejected, not decoded."""
+    from idx2_amd import api
+    data = synth.make_field((64, 64, 64), "float32", seed=4)
+    ds = api.Dataset.encode(ctx, data, brick=(32, 32, 32), nlevels=1, tolerance=1e-3)
+    with pytest.raises(api.Idx2Error):
+        ds.decode(first=(10, 10, 10), last=(5, 5, 5))
+    with pytest.raises(api.Idx2Error):
+        ds.decode(first=(70, 0, 0), last=(80, 10, 10))
+    ds.free()
+
+
 def test_roi_is_crop_of_full_decode(ctx):
     from idx2_amd import api
     print("case roi crop", flush=True)
