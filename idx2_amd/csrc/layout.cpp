@@ -304,24 +304,23 @@ std::string Layout::file_path(const std::string& dir, uint64_t brick, int l) con
 }
 
 void Layout::level_bricks(int l, std::vector<uint64_t>* lin, std::vector<int>* xyz) const {
-  int n = nbricks[l][0] * nbricks[l][1] * nbricks[l][2];
-  std::vector<std::pair<uint64_t, int>> v;
-  v.reserve(n);
-  for (int z = 0; z < nbricks[l][2]; ++z)
-    for (int y = 0; y < nbricks[l][1]; ++y)
-      for (int x = 0; x < nbricks[l][0]; ++x) {
-        int b3[3] = {x, y, z};
-        v.push_back({linear_brick(l, b3), (z * nbricks[l][1] + y) * nbricks[l][0] + x});
-      }
-  std::sort(v.begin(), v.end());
-  lin->resize(n);
-  xyz->resize(3 * n);
-  for (int i = 0; i < n; ++i) {
-    (*lin)[i] = v[i].first;
-    int k = v[i].second;
-    (*xyz)[3 * i] = k % nbricks[l][0];
-    (*xyz)[3 * i + 1] = (k / nbricks[l][0]) % nbricks[l][1];
-    (*xyz)[3 * i + 2] = k / (nbricks[l][0] * nbricks[l][1]);
+  // walk the linear ids in order and keep those inside the level's brick
+  // grid: the result comes out sorted, no per-brick search or sort
+  const int n = nbricks[l][0] * nbricks[l][1] * nbricks[l][2];
+  lin->clear();
+  xyz->clear();
+  lin->reserve(n);
+  xyz->reserve(3 * (size_t)n);
+  const uint64_t span = linear_span(l);
+  const int nb = (int)bstr[l].size();
+  int axis[64];
+  for (int i = 0; i < nb; ++i) axis[i] = bstr[l][i] - 'X';
+  for (uint64_t id = 0; id < span && (int)lin->size() < n; ++id) {
+    int b3[3] = {0, 0, 0};
+    for (int i = 0; i < nb; ++i) b3[axis[i]] = (b3[axis[i]] << 1) | int((id >> (nb - i - 1)) & 1);
+    if (b3[0] >= nbricks[l][0] || b3[1] >= nbricks[l][1] || b3[2] >= nbricks[l][2]) continue;
+    lin->push_back(id);
+    xyz->insert(xyz->end(), b3, b3 + 3);
   }
 }
 
