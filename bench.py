@@ -388,6 +388,8 @@ def main():
                     help="skip the progressive decode sweep")
     ap.add_argument("--no-batched", dest="batched", action="store_false",
                     help="skip the 64-field batched-vs-per-field comparison")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shapes instead of tuning them first")
     ap.add_argument("--no-slice", dest="slice2d", action="store_false",
                     help="skip the 2D float64 8192^2 slice")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -410,6 +412,13 @@ def main():
 
     ctx = api.Context(0)
     cfg = CONFIGS[args.config]
+    tuning = None
+    if args.autotune:
+        log("autotune")
+        t0 = time.perf_counter()
+        shapes, best, dflt = api.autotune(ctx)
+        tuning = {"shapes": shapes, "probe_ms_tuned": round(best, 3), "probe_ms_default": round(dflt, 3),
+                  "tuning_s": round(time.perf_counter() - t0, 2)}
     log(f"plain path: {cfg['workload']}")
     m = plain_bench(ctx, torch, cfg, args.steps, args.warmup, args.seed)
     traffic = measured_traffic(m["dom_name"]) if args.config == "c2" else None
@@ -441,6 +450,8 @@ def main():
                      "kernel_ms": round(m["dom_ms"], 3), "launches": m["dom_launches"],
                      "algorithmic_bytes": int(m["dom_bytes"])},
     }
+    if tuning is not None:
+        res["autotune"] = tuning
     if args.slice2d and args.config == "c2":
         log("2D slice")
         s2 = plain_bench(ctx, torch, CONFIGS["c2b"], 2, 1, args.seed)

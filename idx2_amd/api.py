@@ -51,12 +51,17 @@ class Stats(ctypes.Structure):
                 ("chunk_bytes", ctypes.c_int64), ("exp_bytes", ctypes.c_int64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("enc_wg_per_cu", ctypes.c_int32), ("dec_wg_per_cu", ctypes.c_int32), ("walk_batch", ctypes.c_int32)]
+
+
 EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_encode",
            "idx2g_dataset_write", "idx2g_open", "idx2g_dataset_free", "idx2g_dataset_info",
            "idx2g_output_dims", "idx2g_decode", "idx2g_last_stats", "idx2g_dataset_chunk_bytes",
            "idx2g_synth_field", "idx2g_server_create", "idx2g_server_submit", "idx2g_server_wait",
            "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint", "idx2g_encode_batch",
-           "idx2g_decode_batch", "idx2g_validate_params"]
+           "idx2g_decode_batch", "idx2g_validate_params", "idx2g_autotune", "idx2g_set_tuning",
+           "idx2g_get_tuning"]
 
 _lib = None
 
@@ -104,6 +109,10 @@ def lib():
         L.idx2g_server_destroy.argtypes = [vp]
         L.idx2g_decode_footprint.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64)]
         L.idx2g_validate_params.argtypes = [ctypes.POINTER(EncodeParams)]
+        L.idx2g_autotune.argtypes = [vp, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]
+        L.idx2g_set_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(Tuning)]
+        L.idx2g_get_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(Tuning)]
         L.idx2g_encode_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(EncodeParams), ctypes.POINTER(vp),
                                          ctypes.c_int, ctypes.POINTER(vp)]
         L.idx2g_decode_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(Query),
@@ -287,6 +296,26 @@ class Dataset:
         _check(lib().idx2g_dataset_chunk_bytes(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                                ctypes.byref(n)))
         return buf
+
+
+def autotune(ctx):
+    """Pick launch shapes for ctx's device by timing probe encodes/decodes
+    (idx2g_autotune); returns (shapes, tuned ms, default ms)."""
+    t = Tuning()
+    best, dflt = ctypes.c_double(), ctypes.c_double()
+    _check(lib().idx2g_autotune(ctx._h, ctypes.byref(t), ctypes.byref(best), ctypes.byref(dflt)))
+    return {k: getattr(t, k) for k, _ in Tuning._fields_}, best.value, dflt.value
+
+
+def set_tuning(device=0, enc_wg_per_cu=0, dec_wg_per_cu=0, walk_batch=0):
+    t = Tuning(enc_wg_per_cu, dec_wg_per_cu, walk_batch)
+    _check(lib().idx2g_set_tuning(device, ctypes.byref(t)))
+
+
+def get_tuning(device=0):
+    t = Tuning()
+    _check(lib().idx2g_get_tuning(device, ctypes.byref(t)))
+    return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
 
 def validate_params(**kw):

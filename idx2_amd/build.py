@@ -9,7 +9,7 @@ LIB = os.path.join(OUT_DIR, "libidx2g.so")
 SRC = [os.path.join(HERE, "csrc", f) for f in ("idx2g.hip", "layout.cpp", "common.cpp")]
 DEPS = SRC + [os.path.join(HERE, "csrc", f) for f in (
     "device.hpp", "kernels.hpp", "brick32.hpp", "code32.hpp", "dec32.hpp", "host.hpp", "layout.hpp", "encode_kernels.hip",
-    "decode_kernels.hip", "encode.inc", "files.inc", "decode.inc", "api.inc", "server.inc")] + [os.path.join(ROOT, "include", "idx2g.h")]
+    "decode_kernels.hip", "encode.inc", "files.inc", "decode.inc", "api.inc", "server.inc", "tune.inc")] + [os.path.join(ROOT, "include", "idx2g.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

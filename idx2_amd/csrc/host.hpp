@@ -67,6 +67,9 @@ namespace idx2g {
 void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, int line);
 
+// Launch shapes of the current device (set by the autotuner or the caller).
+idx2g_tuning tuning_of(int device);
+
 // Device memory through a per-device cache of freed blocks: repeated
 // encode/decode calls reuse their large buffers instead of paying
 // hipMalloc/hipFree each time.  Blocks are only handed back after the

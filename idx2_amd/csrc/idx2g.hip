@@ -115,3 +115,4 @@ static bool full_b32(const Layout& L, const int* b3, const int gd[3]) {
 #include "decode.inc"
 #include "api.inc"
 #include "server.inc"
+#include "tune.inc"

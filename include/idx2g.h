@@ -173,6 +173,25 @@ int idx2g_dataset_chunk_bytes(idx2g_dataset* ds, uint8_t* buf, int64_t* nbytes);
 int idx2g_synth_field(idx2g_ctx* ctx, void* dev_out, const int32_t dims[3], int32_t dtype,
                       const double* params, uint64_t noise_seed, int32_t z0, int32_t z1);
 
+/* ---- launch-shape autotuner ----
+ * No counterpart in the reference.  The brick shape is fixed by the format
+ * and the on-chip kernels run one 1024-thread workgroup per CU; what is left
+ * to pick per device is how many workgroups of the generic kernels share a
+ * CU and how many full bricks one decode walk / on-chip launch pair takes.
+ * Results never depend on these (the tests check that); only time does. */
+typedef struct {
+  int32_t enc_wg_per_cu; /* generic encode kernel: resident workgroups per CU (0: as many as its LDS allows) */
+  int32_t dec_wg_per_cu; /* generic decode kernel: same */
+  int32_t walk_batch;    /* decode: full bricks per walk + on-chip launch pair (0: 8192) */
+} idx2g_tuning;
+/* Times synthetic 2D and 3D encodes/decodes on ctx's device for each
+ * candidate shape, keeps the fastest for that device (process-wide) and
+ * returns it.  best_ms / default_ms: the tuned and untuned times of the
+ * probe workload (may be NULL). */
+int idx2g_autotune(idx2g_ctx* ctx, idx2g_tuning* out, double* best_ms, double* default_ms);
+int idx2g_set_tuning(int hip_device, const idx2g_tuning* t);
+int idx2g_get_tuning(int hip_device, idx2g_tuning* t);
+
 /* ---- decode serving loop ----
  * No counterpart in the reference (its decoder serves one query per call,
  * Source/Core/idx2Decode.cpp:326-492, Source/idx2.h:43-44): a queue of decode
