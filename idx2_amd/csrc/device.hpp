@@ -102,15 +102,89 @@ __device__ __forceinline__ void ilift_line(double* __restrict__ L, long sd, int 
   }
 }
 
-// One lifting pass of a whole grid along axis d, lines spread over the
-// threads of the block.  The other two axes follow the reference macros:
-// X -> loops (Y, Z), Y -> (X, Z), Z -> (X, Y); coordinates clamp to M.
+// The same two liftings with one line point per lane: lane i holds point i
+// of its line (i = D: the extrapolated point of an even-length line), lanes
+// of one line are consecutive.  Every lane of the wave calls these (DPP
+// reads); lanes outside a line pass any value and are ignored.  The
+// operation order per point is flift_line's / ilift_line's.
+__device__ __forceinline__ double dpp_prev_f64(double v) {  // lane - 1 (DPP wave_shr:1)
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x138, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double dpp_next_f64(double v) {  // lane + 1 (DPP wave_shl:1)
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x130, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double flift_lanes(double v, int i, int D) {
+  const bool ext = (D & 1) == 0;
+  const int hi_odd = ext ? D - 3 : D - 2;  // last odd point of the predict/update loops
+  double l = dpp_prev_f64(v);
+  const double ll = dpp_prev_f64(l);
+  if (ext && i == D) v = 2 * l - ll;  // L[x0] = 2 B - A
+  l = dpp_prev_f64(v);
+  double r = dpp_next_f64(v);
+  if ((i & 1) && i <= hi_odd) v = v - (l + r) / 2;
+  if (ext && i == D - 1) v = 0;
+  l = dpp_prev_f64(v);
+  r = dpp_next_f64(v);
+  if (!(i & 1) && i <= D - 1) {
+    if (i >= 1 && i - 1 <= hi_odd) v = v + l / 4;
+    if (i + 1 <= hi_odd) v = v + r / 4;
+  }
+  return v;
+}
+
+__device__ __forceinline__ double ilift_lanes(double v, int i, int D) {
+  const bool ext = (D & 1) == 0;
+  const int hi_odd = ext ? D - 3 : D - 2;
+  double l = dpp_prev_f64(v), r = dpp_next_f64(v);
+  if (!(i & 1) && i <= D - 1) {
+    if (i >= 1 && i - 1 <= hi_odd) v = v - l / 4;
+    if (i + 1 <= hi_odd) v = v - r / 4;
+  }
+  l = dpp_prev_f64(v);
+  r = dpp_next_f64(v);
+  if (ext && i == D - 1) v = (r + l) / 2;  // L[x1] = (L[x0] + L[x2]) / 2
+  if ((i & 1) && i <= hi_odd) v = v + (l + r) / 2;
+  return v;
+}
+
+// One lifting pass of a whole grid along axis d.  The other two axes follow
+// the reference macros: X -> loops (Y, Z), Y -> (X, Z), Z -> (X, Y);
+// coordinates clamp to M.  Lines of at most 64 points that never reach the
+// storage limit run one point per lane, several lines per wave; others one
+// line per thread.
 __device__ __forceinline__ void lift_pass(const BrickConst& k, double* F, bool fwd, int d, const DGrid& g,
                                           const int M[3]) {
   if (g.d[d] == 1) return;
   const int a = d == 0 ? 1 : 0, b = d == 2 ? 1 : 2;
   const long st[3] = {1, (long)k.N[0], (long)k.N[0] * k.N[1]};
   const int nl = g.d[a] * g.d[b];
+  const int D = g.d[d], Dn = D + ((D & 1) == 0);
+  if (Dn <= 64 && g.f[d] + g.s[d] * (Dn - 1) <= M[d]) {
+    const int per = 64 / Dn, lane = __lane_id(), slot = lane / Dn, i = lane - slot * Dn;
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int j0 = wave * per; j0 < nl; j0 += nw * per) {
+      const int li = j0 + slot;
+      const bool act = slot < per && li < nl;
+      double* q = F;
+      if (act) {
+        const int ia = li % g.d[a], ib = li / g.d[a];
+        const int ca = min(g.f[a] + g.s[a] * ia, M[a]);
+        const int cb = min(g.f[b] + g.s[b] * ib, M[b]);
+        q = F + ca * st[a] + cb * st[b] + (long)(g.f[d] + g.s[d] * i) * st[d];
+      }
+      double v = act ? *q : 0.0;
+      v = fwd ? flift_lanes(v, i, D) : ilift_lanes(v, i, D);
+      if (act) *q = v;
+    }
+    return;
+  }
   for (int li = threadIdx.x; li < nl; li += blockDim.x) {
     int ia = li % g.d[a], ib = li / g.d[a];
     int ca = min(g.f[a] + g.s[a] * ia, M[a]);
