@@ -317,6 +317,36 @@ def progressive_bench(ctx, torch, seed, n_roi=32):
     return {"encode_ms": round(enc_ms, 2), "queries": len(rows), "sweep": summary}
 
 
+def host_path_bench(ctx, torch, cfg, seed, reps=2):
+    """The same encode + decode with the grid and the output in (pageable) host
+    memory: the rate a caller that hands over host buffers sees (PCIe copies
+    included)."""
+    from idx2_amd import api
+    nx, ny, nz = cfg["dims"]
+    dt = cfg["dtype"]
+    tdt = torch.float64 if dt == "float64" else torch.float32
+    buf = torch.empty(nx * ny * nz, dtype=tdt, device="cuda")
+    ctx.synth_field(buf.data_ptr(), cfg["dims"], dt, seed)
+    host = buf.cpu().numpy().reshape(nz, ny, nx)
+    del buf
+    raw = host.nbytes
+    kw = dict(brick=cfg["brick"], nlevels=cfg["nlevels"], tolerance=cfg["tolerance"])
+    enc, dec = [], []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        ds = api.Dataset.encode(ctx, host, name="h", field="f", **kw)
+        t1 = time.perf_counter()
+        out = ds.decode()
+        t2 = time.perf_counter()
+        ds.free()
+        enc.append(t1 - t0)
+        dec.append(t2 - t1)
+    e, d = float(np.mean(enc[1:])), float(np.mean(dec[1:]))
+    ok = bool(np.abs(out.astype(np.float64) - host.astype(np.float64)).max() < 1.0)
+    return {"encode_ms": round(e * 1e3, 2), "decode_ms": round(d * 1e3, 2), "gbps": round(raw / (e + d) / 1e9, 3),
+            "note": "host input / host output, pageable memory, PCIe copies included", "decoded": ok}
+
+
 def plain_bench(ctx, torch, cfg, steps, warmup, seed):
     """Encode + full decode of one grid resident in HBM, `steps` timed rounds."""
     from idx2_amd import api
@@ -390,6 +420,8 @@ def main():
                     help="skip the 64-field batched-vs-per-field comparison")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shapes instead of tuning them first")
+    ap.add_argument("--no-host-path", dest="host_path", action="store_false",
+                    help="skip the host-buffer (PCIe-inclusive) encode/decode")
     ap.add_argument("--no-slice", dest="slice2d", action="store_false",
                     help="skip the 2D float64 8192^2 slice")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -459,6 +491,9 @@ def main():
                            "encode_ms": round(s2["enc_ms"], 3), "decode_ms": round(s2["dec_ms"], 3),
                            "kernel": s2["dom_name"], "kernel_ms": round(s2["dom_ms"], 3),
                            "kernel_achieved_gbps": round(s2["achieved"], 2), "max_abs_err": s2["err"]}
+    if args.host_path and args.config == "c2":
+        log("host-buffer path")
+        res["host_path"] = host_path_bench(ctx, torch, cfg, args.seed)
     if args.serve_requests > 0:
         log(f"serving loop: {args.serve_requests} requests at {args.serve_rate}/s")
         res["serving"] = serve_bench(ctx, torch, args.serve_requests, args.serve_rate, args.seed)

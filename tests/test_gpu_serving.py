@@ -93,3 +93,24 @@ def test_budget_serialises(resident):
     assert info["peak_running"] == 1
     for (_, q, want), g in zip(jobs, got):
         assert np.array_equal(g.view(np.uint8), want.reshape(-1).view(np.uint8)), q
+
+
+def test_bad_request_is_rejected_at_submit(resident):
+    """A query that is empty after clamping is refused by submit with an
+    error code; the server keeps serving the next requests."""
+    import torch
+    from idx2_amd import api
+    ctx, sets = resident
+    ds, dims = sets[0]
+    server = api.Server(0, hbm_budget=1 << 30, slots=2)
+    buf = torch.empty(16, dtype=torch.float32, device="cuda")
+    with pytest.raises(api.Idx2Error):
+        server.submit(ds, buf.data_ptr(), first=(10, 10, 10), last=(5, 5, 5))
+    q = dict(first=(0, 0, 0), last=(7, 7, 7))
+    want = ds.decode(**q)
+    out = torch.empty(want.size, dtype=torch.float32, device="cuda")
+    rid = server.submit(ds, out.data_ptr(), **q)
+    server.wait(rid)
+    server.close()
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint8), want.reshape(-1).view(np.uint8))
