@@ -568,30 +568,6 @@ __global__ void k_scatter(const uint8_t* __restrict__ slab, const ScatterSeg* __
   for (u64 i = nv * 16 + threadIdx.x; i < sg.n; i += blockDim.x) d[i] = s[i];
 }
 
-// Byte copy by one thread: aligned 32-bit destination words, each assembled
-// from two aligned source words (sources are padded by >= 4 bytes).
-__device__ inline void copy_bytes(uint8_t* d, const uint8_t* s, uint32_t n) {
-  uint32_t i = 0;
-  const uint32_t head = min(n, (4u - (uint32_t)((uintptr_t)d & 3)) & 3u);
-  for (; i < head; ++i) d[i] = s[i];
-  const uint32_t nw = (n - i) >> 2;
-  uint32_t* dw = reinterpret_cast<uint32_t*>(d + i);
-  const uint8_t* sp = s + i;
-  const uint32_t sh = (uint32_t)((uintptr_t)sp & 3);
-  const uint32_t* sw = reinterpret_cast<const uint32_t*>(sp - sh);
-  uint32_t k = 0;
-  for (; k + 8 <= nw; k += 8) {
-    uint32_t t[9];
-#pragma unroll
-    for (int j = 0; j < 9; ++j) t[j] = sw[k + j];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) dw[k + j] = __builtin_amdgcn_alignbyte(t[j + 1], t[j], sh);
-  }
-  for (; k < nw; ++k) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
-  i += nw * 4;
-  for (; i < n; ++i) d[i] = s[i];
-}
-
 // Byte copy by a whole wave: aligned 32-bit destination words, each lane
 // assembling its word from two aligned source words (sources are padded by
 // >= 4 bytes).

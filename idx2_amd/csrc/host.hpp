@@ -38,11 +38,6 @@ struct idx2g_ctx {
   int ncu = 256;
   // timing events
   hipEvent_t ev[8];
-  // kernel argument blocks: a ring of pinned host slots mirrored on the
-  // device (kernels read their arguments through a pointer)
-  uint8_t* h_args = nullptr;
-  uint8_t* d_args = nullptr;
-  int arg_slot = 0;
   // pinned staging of batched uploads (ev[6] marks the last transfer out of it)
   uint8_t* h_stage = nullptr;
   size_t stage_cap = 0;

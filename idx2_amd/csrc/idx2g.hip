@@ -30,24 +30,6 @@ static int dalloc(T** p, size_t n) {
   HIPCHK(dev_alloc((void**)p, n * sizeof(T)));
   return IDX2G_OK;
 }
-// Copy a kernel argument block into the next slot of the context's ring and
-// return its device address.  The ring is drained (stream synchronised)
-// before a slot is reused.
-static constexpr size_t kArgSlot = 2048, kArgSlots = 256;
-template <class T>
-static int stage_args(idx2g_ctx* ctx, const T& a, const T** out) {
-  static_assert(sizeof(T) <= kArgSlot, "argument block too large");
-  if (ctx->arg_slot == (int)kArgSlots) {
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    ctx->arg_slot = 0;
-  }
-  const size_t off = (size_t)ctx->arg_slot++ * kArgSlot;
-  memcpy(ctx->h_args + off, &a, sizeof(T));
-  HIPCHK(hipMemcpyAsync(ctx->d_args + off, ctx->h_args + off, sizeof(T), hipMemcpyHostToDevice, ctx->stream));
-  *out = (const T*)(ctx->d_args + off);
-  return IDX2G_OK;
-}
-
 #define TRY(x)                  \
   do {                          \
     int rc_ = (x);              \
