@@ -299,14 +299,17 @@ __device__ __forceinline__ void stage_subband_odd(const double* Z1, const Lds& S
   stage_faces(S, V, p, hx, hy, 1);
 }
 
-__device__ __forceinline__ uint32_t& lno_at(CodeLds& C, int npl, int b, int pp) { return C.pool[b * npl + pp]; }
-__device__ __forceinline__ u64* plane_at(CodeLds& C, u64* XG, int npl, int b, int pp) {
+template <class CL>
+__device__ __forceinline__ uint32_t& lno_at(CL& C, int npl, int b, int pp) { return C.pool[b * npl + pp]; }
+template <class CL>
+__device__ __forceinline__ u64* plane_at(CL& C, u64* XG, int npl, int b, int pp) {
   return npl == 32 ? reinterpret_cast<u64*>(C.pool + kMaxBlocks * 32) + b * 32 + pp : XG + b * 64 + pp;
 }
 
 // Coefficient c of block slot qq (coefficient order, then negabinary),
 // ForwardShuffle (Zfp.h:534-559).
-__device__ __forceinline__ u64 coef_u(const i64* Tq, const CodeLds& C, int nd, int c) {
+template <class CL>
+__device__ __forceinline__ u64 coef_u(const i64* Tq, const CL& C, int nd, int c) {
   const int nv = 1 << (2 * nd);
   if (c >= nv) return 0;
   const int src = nd == 3 ? C.p3[c] : nd == 2 ? C.p2[c] : c;
@@ -316,8 +319,9 @@ __device__ __forceinline__ u64 coef_u(const i64* Tq, const CodeLds& C, int nd, i
 // Plane mode after the transpose: lane j holds plane word X of block slot
 // `slot` (of the wave's group) and plane pp.  seg: lanes per block (32 for
 // f32 pairs, 64 for f64).
+template <class CL>
 __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bool blk_ok, int np, int nd, int pp,
-                                           CodeLds& C, u64* XG, int npl) {
+                                           CL& C, u64* XG, int npl) {
   const int k = lane & (seg - 1);
   const bool coded = blk_ok && pp < np;
   const int t = (coded && X) ? 64 - clz64(X) : 0;
@@ -333,9 +337,11 @@ __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bo
 
 // Block stage of one subband staged in C.v.V (dims dx, dy, ...): per block
 // exponent, planes and their (length, N); 4 blocks per wave.
-template <int DT>
-__device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint32_t* blocks, int nblk, CodeLds& C,
-                            const Pos& p, int dense, u64* XG) {
+// kSeg: the blocks belong to several bricks (block z = brick slot of the
+// group, seg_dense[z] its dense id): exponent rows and key ranges per brick.
+template <int DT, bool kSeg = false, class CL = CodeLds>
+__device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, int dy, const uint32_t* blocks, int nblk, CL& C,
+                            const Pos& p, int dense, u64* XG, const int* seg_dense = nullptr) {
   const BrickConst& k = a.k;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   i64* T = &C.T[p.w][0][0];
@@ -410,7 +416,8 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
       C.np[b] = (unsigned char)np;
       if (!skip) {
         // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
-        const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (DT ? 2 : 1);
+        const int bdense = kSeg ? seg_dense[o3[2]] : dense;
+        const long long row = a.exp_off[sb] + ((long long)bdense * a.ncoded[sb] + ci) * (DT ? 2 : 1);
         if (DT) {
           const uint16_t e16 = (uint16_t)(emax + 1023);
           a.exps[row] = (uint8_t)(e16 & 0xff);
@@ -419,8 +426,13 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
           a.exps[row] = (uint8_t)(emax + 127);
         }
         if (np > 0) {
-          atomicMax(&C.khi, 63 + emax + kBpBiasDev);
-          atomicMin(&C.klo, stop + emax + kBpBiasDev);
+          if constexpr (kSeg) {
+            atomicMax(&C.khi_s[o3[2]], 63 + emax + kBpBiasDev);
+            atomicMin(&C.klo_s[o3[2]], stop + emax + kBpBiasDev);
+          } else {
+            atomicMax(&C.khi, 63 + emax + kBpBiasDev);
+            atomicMin(&C.klo, stop + emax + kBpBiasDev);
+          }
         }
       }
     }

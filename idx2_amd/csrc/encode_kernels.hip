@@ -10,6 +10,7 @@
 #include "kernels.hpp"
 #include "brick32.hpp"
 #include "code32.hpp"
+#include "code2d.hpp"
 
 namespace idx2g {
 
@@ -427,6 +428,156 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
     b32::weigh(c, S, p, w8);
     b32::code_brick<DT>(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
                     reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride), s_blocks, cur);
+    __syncthreads();
+  }
+}
+
+// Full 2D 32x32 bricks, 32 per workgroup iteration (one per 32-lane
+// half-wave, brick2d.hpp), coded kGroup bricks per pass (code2d.hpp).  Work
+// item g of job j is the batch of bricks [32 (g - start[j]), ...) of its list.
+template <int DT>
+__global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restrict__ table,
+                                                        const int* __restrict__ start, int njobs) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ double w4[4];
+  __shared__ uint32_t s_blocks[4 * 32];
+  __shared__ int s_dense[b2d::kSlots];
+  __shared__ int s_off[4];
+  __shared__ b32::ArenaCursor cur;
+  b2d::Lds2& S = *reinterpret_cast<b2d::Lds2*>(lds_raw);
+  b2d::CodeLds2& C = S.u.code;
+  if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0};
+  const int total = start[njobs];
+  int cj = -1;
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const EncArgs& a = table[j];
+    const BrickConst& k = a.k;
+    if (j != cj) {  // block tables, weights and park offsets of the job's subbands
+      __syncthreads();
+      for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) {
+        const int sb = i >> 5, b = i & 31;
+        s_blocks[i] = b < a.blk_n[sb] ? a.blocks[a.blk_off[sb] + b] : 0u;
+      }
+      if (threadIdx.x == 0) {
+        int off = 0;
+        for (int sb = 0; sb < 4; ++sb) {
+          const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
+          w4[hx + 2 * hy] = (sb == 0 && !a.coarsest) ? 1.0 : a.wfwd[sb];
+          s_off[hx + 2 * hy] = off;
+          off += (17 - hx) * (17 - hy) * b2d::kSlots;
+        }
+      }
+      cj = j;
+      __syncthreads();
+    }
+    const int batch = g - start[j];
+    const int nvalid = min(b2d::kSlots, a.nb - batch * b2d::kSlots);
+    const b2d::Pos2 p = b2d::pos2();
+    const bool has = p.slot < nvalid;
+    const int dense = has ? a.list[batch * b2d::kSlots + p.slot] : 0;
+    if (p.x == 0) s_dense[p.slot] = dense;
+    const int bx = a.xyz[3 * dense], by = a.xyz[3 * dense + 1];
+    double c[b32::kN];
+    {
+      const unsigned row = (unsigned)a.gdims[0];
+      const size_t base = (size_t)(by * 32) * a.gdims[0] + (size_t)bx * 32 + p.x;
+      if (a.in_dtype) {
+        const double* src = (const double*)a.input + base;
+#pragma unroll
+        for (int y = 0; y < 32; ++y) c[y] = has ? src[(size_t)y * row] : 0.0;
+      } else {
+        const float* src = (const float*)a.input + base;
+#pragma unroll
+        for (int y = 0; y < 32; ++y) c[y] = has ? (double)src[(size_t)y * row] : 0.0;
+      }
+    }
+    c[32] = 0.0;
+    if (a.level == 0) {
+      double mn = c[0], mx = c[0];
+#pragma unroll
+      for (int y = 1; y < 32; ++y) {
+        mn = c[y] < mn ? c[y] : mn;
+        mx = c[y] > mx ? c[y] : mx;
+      }
+      for (int o = 16; o > 0; o >>= 1) {  // over the slot's 32 lanes
+        const double omn = __shfl_xor(mn, o, 64), omx = __shfl_xor(mx, o, 64);
+        mn = omn < mn ? omn : mn;
+        mx = omx > mx ? omx : mx;
+      }
+      if (has && p.x == 0) {
+        a.minmax[2 * dense] = mn;
+        a.minmax[2 * dense + 1] = mx;
+      }
+    }
+    __syncthreads();
+    b2d::extrapolate(c, S.ex, S.u.sub, p);
+    b2d::forward(c, S.ex, p);
+    // subband 0 interior into the next level's grid (EncodeBrick, idx2Encode.cpp:303-333)
+    if (!a.coarsest && has && !(p.x & 1)) {
+      double* dst = a.parents + (size_t)(by * 16) * a.pdims[0] + (size_t)bx * 16 + (p.x >> 1);
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) dst[(size_t)jj * a.pdims[0]] = c[2 * jj];
+    }
+    // subband weights by parity (x = 32 is even), then park every slot's
+    // subbands: P[off(hx, hy) + (slot * dy + j) * dx + i] = (2i + hx, 2j + hy)
+    double* P = a.scratch + (size_t)blockIdx.x * a.wstride + b2d::kParkOff;
+    {
+      const double we = w4[p.x & 1], wo = w4[(p.x & 1) + 2];
+#pragma unroll
+      for (int y = 0; y < b32::kN; ++y) c[y] = c[y] * ((y & 1) ? wo : we);
+      for (int t = p.tid; t < b2d::kSlots * b32::kN; t += b2d::kTh) {
+        const int y = t % b32::kN;
+        S.ex[t] = S.ex[t] * w4[2 * (y & 1)];
+      }
+    }
+#pragma unroll
+    for (int hy = 0; hy < 2; ++hy) {
+      const int dy = 17 - hy;
+      if ((p.x & 1) == 0) {
+        double* q = P + s_off[2 * hy] + (p.slot * dy) * 17 + (p.x >> 1);
+#pragma unroll
+        for (int jj = 0; jj < 17 - hy; ++jj) q[jj * 17] = c[2 * jj + hy];
+      } else {
+        double* q = P + s_off[1 + 2 * hy] + (p.slot * dy) * 16 + (p.x >> 1);
+#pragma unroll
+        for (int jj = 0; jj < 17 - hy; ++jj) q[jj * 16] = c[2 * jj + hy];
+      }
+    }
+    __syncthreads();  // EX weighted
+    for (int t = p.tid; t < b2d::kSlots * 33; t += b2d::kTh) {  // x = 32 column (hx = 0)
+      const int s2 = t / 33, r = t - s2 * 33, hy = r & 1, jj = r >> 1, dy = 17 - hy;
+      P[s_off[2 * hy] + (s2 * dy + jj) * 17 + 16] = S.ex[s2 * b32::kN + r];
+    }
+    if (p.tid < 64) C.p3[p.tid] = a.perm.p3[p.tid];
+    if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
+    if (p.tid < 256) {
+      uint32_t e = 0;
+      int o = 0;
+      for (int i = 0; i < 8; ++i) {
+        if ((p.tid >> i) & 1) { e |= 3u << o; o += 2; }
+        else o += 1;
+      }
+      C.e8[p.tid] = (uint16_t)e;
+    }
+    __syncthreads();
+    b32::Pos pb;
+    pb.tid = p.tid;
+    pb.lane = p.lane;
+    pb.w = p.w;
+    pb.x = p.x;
+    pb.y = p.slot;
+    pb.orow = p.lane >= 32;
+    u64* XG = reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride);
+    for (int sb = 0; sb < 4; ++sb) {
+      const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
+      const int area = (17 - hx) * (17 - hy);
+      for (int s0 = 0; s0 < nvalid; s0 += b2d::kGroup) {
+        const int G = min(b2d::kGroup, nvalid - s0);
+        b2d::code_group<DT>(a, sb, G, s_dense + s0, P + s_off[hx + 2 * hy] + s0 * area, C, pb, XG, s_blocks + sb * 32,
+                            a.blk_n[sb], cur);
+      }
+    }
     __syncthreads();
   }
 }

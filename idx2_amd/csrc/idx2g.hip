@@ -90,6 +90,18 @@ static bool full_b32(const Layout& L, const int* b3, const int gd[3]) {
   return true;
 }
 
+// 2D bricks the on-chip 2D kernel takes: brick 32x32x1 of a 2D grid, data
+// extent 32 on both axes.
+static bool full_2d32(const Layout& L, const int* b3, const int gd[3]) {
+  if (L.brick[0] != 32 || L.brick[1] != 32 || L.brick[2] != 1 || gd[2] != 1) return false;
+  for (int d = 0; d < 2; ++d)
+    if ((b3[d] + 1) * 32 > gd[d]) return false;
+  for (int s = 0; s < L.nsb; ++s)  // the group coder holds kGroup bricks' blocks (<= 125)
+    for (int l = 0; l < L.nlevels; ++l)
+      if ((int)L.blocks[l][s].size() * b2d::kGroup > kMaxBlocks) return false;
+  return true;
+}
+
 }  // namespace idx2g
 
 #include "encode.inc"
