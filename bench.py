@@ -385,14 +385,15 @@ def plain_bench(ctx, torch, cfg, steps, warmup, seed):
     enc_ms = np.array([e["total_ms"] for e in enc])
     dec_ms = np.array([d["total_ms"] for d in dec])
     # dominant kernel: the level-0 brick launches of encode or decode, whichever is longer
-    on_chip = tuple(cfg["brick"]) == (32, 32, 32) and nz > 1
+    brick = tuple(cfg["brick"])
+    kind = "b32" if brick == (32, 32, 32) and nz > 1 else "2d" if brick == (32, 32, 1) and nz == 1 else None
     dom_e = float(np.mean([e["dominant_ms"] for e in enc]))
     dom_d = float(np.mean([d["dominant_ms"] for d in dec]))
     if dom_e >= dom_d:
-        name = "idx2g::k_encode_b32 (level 0)" if on_chip else "idx2g::k_encode_level (level 0)"
+        name = f"idx2g::k_encode_{kind} (level 0)" if kind else "idx2g::k_encode_level (level 0)"
         dom_ms, dom_bytes, launches = dom_e, float(np.mean([e["dominant_bytes"] for e in enc])), 1
     else:
-        name = ("idx2g::k_walk_b32 + idx2g::k_decode_b32 (level 0)" if on_chip
+        name = (f"idx2g::k_walk_b32 + idx2g::k_decode_{kind} (level 0)" if kind
                 else "idx2g::k_decode_level (level 0)")
         dom_ms, dom_bytes = dom_d, float(np.mean([d["dominant_bytes"] for d in dec]))
         launches = int(np.mean([d["dominant_launches"] for d in dec]))

@@ -8,7 +8,8 @@ OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libidx2g.so")
 SRC = [os.path.join(HERE, "csrc", f) for f in ("idx2g.hip", "layout.cpp", "common.cpp")]
 DEPS = SRC + [os.path.join(HERE, "csrc", f) for f in (
-    "device.hpp", "kernels.hpp", "brick32.hpp", "code32.hpp", "dec32.hpp", "host.hpp", "layout.hpp", "encode_kernels.hip",
+    "device.hpp", "kernels.hpp", "brick32.hpp", "code32.hpp", "dec32.hpp", "brick2d.hpp", "code2d.hpp",
+    "dec2d.hpp", "host.hpp", "layout.hpp", "encode_kernels.hip",
     "decode_kernels.hip", "encode.inc", "files.inc", "decode.inc", "api.inc", "server.inc", "tune.inc")] + [os.path.join(ROOT, "include", "idx2g.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -12,6 +12,7 @@
 
 #include "kernels.hpp"
 #include "dec32.hpp"
+#include "dec2d.hpp"
 
 namespace idx2g {
 
@@ -415,6 +416,164 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
       double* dst = a.cgrid + ((size_t)(b3[2] * 32) * a.cgdims[1] + (b3[1] * 32 + p.y)) * a.cgdims[0] + (b3[0] * 32 + p.x);
 #pragma unroll
       for (int z = 0; z < 32; ++z) dst[z * plane] = c[z];
+    }
+    __syncthreads();
+  }
+}
+
+// Full 32x32 bricks of a 2D dataset (dec2d.hpp): 32 bricks per workgroup
+// iteration (brick slot = half-wave), start[] in iterations; the walk
+// (k_walk_b32) has left the plane words in a.xs / a.nbps by piece-local
+// brick index.
+template <int DT>
+__global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restrict__ table,
+                                                        const int* __restrict__ start, int njobs) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  __shared__ uint32_t s_blocks[4 * 32];
+  __shared__ double w4[4];
+  __shared__ int s_b3[b2d::kSlots][2];
+  __shared__ int s_poff[2];  // parking areas of the hy = 1 subbands by hx
+  b2d::DecLds2& D = *reinterpret_cast<b2d::DecLds2*>(lds_raw);
+  const int total = start[njobs];
+  int cj = -1;
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int j = job_of(start, njobs, g);
+    const DecArgs& a = table[j];
+    const BrickConst& k = a.k;
+    if (j != cj) {  // block tables and inverse weights of the job's subbands
+      __syncthreads();
+      for (int i = threadIdx.x; i < 4 * 32; i += blockDim.x) {
+        const int sb = i >> 5, b = i & 31;
+        s_blocks[i] = b < a.blk_n[sb] ? a.blocks[a.blk_off[sb] + b] : 0u;
+      }
+      if (threadIdx.x < 4) {
+        const int sb = threadIdx.x;
+        w4[k.sb[sb].f[0] + 2 * k.sb[sb].f[1]] = (sb == 0 && !a.coarsest) ? 1.0 : a.winv[sb];
+      }
+      if (threadIdx.x < 2) s_poff[threadIdx.x] = threadIdx.x ? 17 * 16 * b2d::kSlots : 0;
+      cj = j;
+      __syncthreads();
+    }
+    const int bi0 = (g - start[j]) * b2d::kSlots;
+    const int nvalid = min(b2d::kSlots, a.nb - bi0);
+    const b2d::Pos2 p = b2d::pos2();
+    if (p.x == 0) {
+      const int bi = bi0 + (p.slot < nvalid ? p.slot : 0);
+      s_b3[p.slot][0] = a.xyz[3 * bi];
+      s_b3[p.slot][1] = a.xyz[3 * bi + 1];
+    }
+    for (int t = p.tid; t < b2d::kSlots * 4 * 32; t += b2d::kTh) {
+      // exponents (EncodeBrickSubbandExponents layout) and decoded plane counts
+      const int s = t >> 7, sb = (t >> 5) & 3, b = t & 31;
+      short e = kSkip;
+      unsigned char nb = 0;
+      if (s < nvalid && b < a.blk_n[sb]) {
+        const uint32_t bw = s_blocks[sb * 32 + b];
+        if (!((bw >> 24) & 1)) {
+          const long long row = a.exp_off[sb] + ((long long)a.dense[bi0 + s] * a.ncoded[sb] + int(bw >> 25)) * (DT ? 2 : 1);
+          e = DT ? (short)(int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023)
+                 : (short)(int(a.exps[row]) - 127);
+        }
+        nb = a.nbps[(size_t)(bi0 + s) * 1024 + sb * 128 + b];
+      }
+      D.E[s][sb][b] = e;
+      D.nbps[s][sb][b] = nb;
+    }
+    if (p.tid < 64) D.p3[p.tid] = a.perm.p3[p.tid];
+    if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
+    for (int t = p.tid; t < b2d::kSlots * b32::kN; t += b2d::kTh) D.ex[t] = 0.0;
+    b32::Pos pb;
+    pb.tid = p.tid;
+    pb.lane = p.lane;
+    pb.w = p.w;
+    pb.x = p.x;
+    pb.y = p.slot;
+    pb.orow = p.lane >= 32;
+    __syncthreads();
+    // One subband of the group's bricks into D.V: zeros, the parent's
+    // subband-0 interior (CopyExtentGrid, Volume.h:1153-1173), the blocks.
+    auto stage = [&](int sb, int s0, int G) {
+      const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
+      const int dx = 17 - hx, dy = 17 - hy;
+      for (int t = p.tid; t < G * dx * dy; t += b2d::kTh) D.V[t] = 0.0;
+      __syncthreads();
+      if (!((a.mask >> sb) & 1)) return;  // subband not decoded: zeros
+      if (sb == 0 && !a.coarsest) {
+        for (int t = p.tid; t < G * 256; t += b2d::kTh) {
+          const int gg = t >> 8, x = t & 15, y = (t >> 4) & 15;
+          D.V[(gg * dy + y) * dx + x] =
+              a.pgrid[(size_t)(s_b3[s0 + gg][1] * 16 + y) * a.pgdims[0] + (s_b3[s0 + gg][0] * 16 + x)];
+        }
+        __syncthreads();
+      }
+      b2d::inverse_blocks_2d<DT>(a, sb, s0, G, dx, dy, D, pb, s_blocks + sb * 32, a.blk_n[sb], bi0, D.V);
+      __syncthreads();
+    };
+    // odd rows (hy = 1) first, parked in the workgroup's scratch slot, so
+    // that only the even rows of the register columns are live meanwhile
+    double* P = a.scratch + (size_t)blockIdx.x * a.wstride;
+    for (int sb = 0; sb < 4; ++sb) {
+      const int hx = k.sb[sb].f[0];
+      if (k.sb[sb].f[1] != 1) continue;
+      for (int s0 = 0; s0 < nvalid; s0 += b2d::kGroup) {
+        const int G = min(b2d::kGroup, nvalid - s0);
+        stage(sb, s0, G);
+        b2d::park_group(D, P + s_poff[hx], p, 17 - hx, 16, s0, G);
+        __syncthreads();
+      }
+    }
+    double c[b32::kN];
+#pragma unroll
+    for (int y = 0; y < b32::kN; y += 2) c[y] = 0.0;
+    for (int sb = 0; sb < 4; ++sb) {
+      const int hx = k.sb[sb].f[0];
+      if (k.sb[sb].f[1] != 0) continue;
+      for (int s0 = 0; s0 < nvalid; s0 += b2d::kGroup) {
+        const int G = min(b2d::kGroup, nvalid - s0);
+        stage(sb, s0, G);
+        b2d::take_subband<0>(c, D, p, hx, s0, G);
+        __syncthreads();
+      }
+    }
+    __threadfence_block();
+    b2d::unpark_odd(c, D.ex, P, s_poff, p);
+    __syncthreads();
+    {  // inverse subband weights by parity (x = 32 is even)
+      const double we = w4[p.x & 1], wo = w4[(p.x & 1) + 2];
+#pragma unroll
+      for (int y = 0; y < b32::kN; ++y) c[y] = c[y] * ((y & 1) ? wo : we);
+      for (int t = p.tid; t < b2d::kSlots * b32::kN; t += b2d::kTh) {
+        const int y = t % b32::kN;
+        D.ex[t] = D.ex[t] * w4[2 * (y & 1)];
+      }
+    }
+    __syncthreads();
+    b2d::inverse(c, D.ex, p);
+    if (p.slot < nvalid) {
+      const int bx = s_b3[p.slot][0], by = s_b3[p.slot][1];
+      if (a.out) {
+        // Crop(OutGrid, BrickGrid) (idx2Decode.cpp:431-445); z = 0 on a 2D grid
+        const int gx = bx * a.B3[0] + p.x * a.bstride;
+        const int xlast = a.og_f[0] + (a.og_d[0] - 1) * a.og_s[0];
+        const bool okx = gx >= a.og_f[0] && gx <= xlast && (gx - a.og_f[0]) % a.og_s[0] == 0;
+        if (okx) {
+          const int ox = (gx - a.og_f[0]) / a.og_s[0];
+          const int ylast = a.og_f[1] + (a.og_d[1] - 1) * a.og_s[1];
+#pragma unroll
+          for (int y = 0; y < 32; ++y) {
+            const int gy = by * a.B3[1] + y * a.bstride;
+            if (gy >= a.og_f[1] && gy <= ylast && (gy - a.og_f[1]) % a.og_s[1] == 0) {
+              const size_t oi = (size_t)((gy - a.og_f[1]) / a.og_s[1]) * a.og_d[0] + ox;
+              if (DT) ((double*)a.out)[oi] = c[y];
+              else ((float*)a.out)[oi] = (float)c[y];
+            }
+          }
+        }
+      } else {
+        double* dst = a.cgrid + (size_t)(by * 32) * a.cgdims[0] + (bx * 32 + p.x);
+#pragma unroll
+        for (int y = 0; y < 32; ++y) dst[(size_t)y * a.cgdims[0]] = c[y];
+      }
     }
     __syncthreads();
   }

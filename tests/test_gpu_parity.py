@@ -125,6 +125,8 @@ EDGE = [
     ((96, 64, 64), "float32", 1, 2.0 ** -23, (32, 32, 32)),
     # 2D, coarse tolerance
     ((64, 64, 1), "float32", 1, 1e-2, (32, 32, 1)),
+    # 2D, full and partial 32x32 bricks on two levels, float32
+    ((160, 100, 1), "float32", 2, 1e-5, (32, 32, 1)),
     # partial bricks on two levels
     ((130, 66, 34), "float32", 2, 1e-5, (32, 32, 32)),
     # 16^3 bricks (generic kernels, bricks kept in LDS)
