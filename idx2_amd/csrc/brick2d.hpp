@@ -72,11 +72,12 @@ __device__ __forceinline__ void ypass(double (&c)[kN], double* ex, const Pos2& p
   }
 }
 
-// Lines of the slots' LDS subgrids (sub[slot][j * 17 + i] = (2i, 2j)) along
-// axis A with index stride SS and D points per line.
+// Lines of the first ns slots' LDS subgrids (sub[slot][j * 17 + i] =
+// (2i, 2j)) along axis A with index stride SS and D points per line.
 template <int MODE, int A, int SS, int D>
-__device__ __forceinline__ void sub_pass(double* sub, const Pos2& p) {
-  constexpr int per = 64 / D, nl = kSlots * D;
+__device__ __forceinline__ void sub_pass(double* sub, const Pos2& p, int ns) {
+  constexpr int per = 64 / D;
+  const int nl = ns * D;
   constexpr int sa = A == 0 ? SS : SS * 17, so = A == 0 ? SS * 17 : SS;
   const int ls = p.lane / D, i = p.lane - ls * D;
   for (int j0 = p.w * per; j0 < nl; j0 += (kTh / 64) * per) {
@@ -91,18 +92,18 @@ __device__ __forceinline__ void sub_pass(double* sub, const Pos2& p) {
 }
 
 template <int MODE, int L>
-__device__ __forceinline__ void sub_level(double* sub, const Pos2& p) {
+__device__ __forceinline__ void sub_level(double* sub, const Pos2& p, int ns) {
   constexpr int SS = 1 << (L - 1);
   constexpr int D = L == 1 ? 17 : L == 2 ? 9 : L == 3 ? 5 : 3;
   if (MODE == kFwd) {
-    sub_pass<MODE, 0, SS, D>(sub, p);
+    sub_pass<MODE, 0, SS, D>(sub, p, ns);
     __syncthreads();
-    sub_pass<MODE, 1, SS, D>(sub, p);
+    sub_pass<MODE, 1, SS, D>(sub, p, ns);
     __syncthreads();
   } else {
-    sub_pass<MODE, 1, SS, D>(sub, p);
+    sub_pass<MODE, 1, SS, D>(sub, p, ns);
     __syncthreads();
-    sub_pass<MODE, 0, SS, D>(sub, p);
+    sub_pass<MODE, 0, SS, D>(sub, p, ns);
     __syncthreads();
   }
 }
@@ -132,20 +133,22 @@ __device__ __forceinline__ void sub_copy(double (&c)[kN], double* ex, double* su
 
 // ExtrapolateCdf53 of full bricks (data extent 32 on both axes): forward
 // levels 0-4 on the growing data grid, then inverse levels 4-0 on 33x33.
-__device__ __forceinline__ void extrapolate(double (&c)[kN], double* ex, double* sub, const Pos2& p) {
+// The coarse levels run on the first ns slots only (the others hold no
+// brick this iteration).
+__device__ __forceinline__ void extrapolate(double (&c)[kN], double* ex, double* sub, const Pos2& p, int ns) {
   xpass<kExt>(c, ex, p);
   ypass<kExt>(c, ex, p);
   __syncthreads();
   sub_copy<true>(c, ex, sub, p);
   __syncthreads();
-  sub_level<kFwd, 1>(sub, p);
-  sub_level<kFwd, 2>(sub, p);
-  sub_level<kFwd, 3>(sub, p);
-  sub_level<kFwd, 4>(sub, p);
-  sub_level<kInv, 4>(sub, p);
-  sub_level<kInv, 3>(sub, p);
-  sub_level<kInv, 2>(sub, p);
-  sub_level<kInv, 1>(sub, p);
+  sub_level<kFwd, 1>(sub, p, ns);
+  sub_level<kFwd, 2>(sub, p, ns);
+  sub_level<kFwd, 3>(sub, p, ns);
+  sub_level<kFwd, 4>(sub, p, ns);
+  sub_level<kInv, 4>(sub, p, ns);
+  sub_level<kInv, 3>(sub, p, ns);
+  sub_level<kInv, 2>(sub, p, ns);
+  sub_level<kInv, 1>(sub, p, ns);
   sub_copy<false>(c, ex, sub, p);
   __syncthreads();
   ypass<kInv>(c, ex, p);

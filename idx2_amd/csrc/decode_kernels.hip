@@ -454,8 +454,8 @@ __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restric
       cj = j;
       __syncthreads();
     }
-    const int bi0 = (g - start[j]) * b2d::kSlots;
-    const int nvalid = min(b2d::kSlots, a.nb - bi0);
+    const int bi0 = (g - start[j]) * a.per;
+    const int nvalid = min(a.per, a.nb - bi0);
     const b2d::Pos2 p = b2d::pos2();
     if (p.x == 0) {
       const int bi = bi0 + (p.slot < nvalid ? p.slot : 0);

@@ -472,10 +472,10 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
       __syncthreads();
     }
     const int batch = g - start[j];
-    const int nvalid = min(b2d::kSlots, a.nb - batch * b2d::kSlots);
+    const int nvalid = min(a.per, a.nb - batch * a.per);
     const b2d::Pos2 p = b2d::pos2();
     const bool has = p.slot < nvalid;
-    const int dense = has ? a.list[batch * b2d::kSlots + p.slot] : 0;
+    const int dense = has ? a.list[batch * a.per + p.slot] : 0;
     if (p.x == 0) s_dense[p.slot] = dense;
     const int bx = a.xyz[3 * dense], by = a.xyz[3 * dense + 1];
     double c[b32::kN];
@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
       }
     }
     __syncthreads();
-    b2d::extrapolate(c, S.ex, S.u.sub, p);
+    b2d::extrapolate(c, S.ex, S.u.sub, p, nvalid);
     b2d::forward(c, S.ex, p);
     // subband 0 interior into the next level's grid (EncodeBrick, idx2Encode.cpp:303-333)
     if (!a.coarsest && has && !(p.x & 1)) {

@@ -48,6 +48,7 @@ struct EncArgs {
   const int* xyz;        // [nb_level][3]
   const int* list;       // dense ids to process, or null for [b0, b0 + nb)
   int nb, b0;
+  int per;               // 2D kernels: bricks per workgroup iteration (1..32)
   double* scratch;       // [gridDim.x][wstride] (generic kernel: work volume; b32: parked z-odd half)
   int wstride;           // doubles per workgroup slot of scratch
   int lds_blocks;        // generic kernel: blocks per subband its LDS layout holds (launch-wide)
@@ -126,6 +127,7 @@ struct DecArgs {
   const u64* lin;        // [n] linear ids
   const int* dense;      // [n] dense ids (exponent rows)
   int nb;
+  int per;               // 2D kernels: bricks per workgroup iteration (1..32)
   double* scratch;       // per workgroup slot: work volume (generic kernel), parked z-odd half (b32)
   int wstride;           // doubles per workgroup slot
   int lds_blocks;        // generic kernel: blocks per subband its LDS layout holds (launch-wide)
