@@ -393,7 +393,7 @@ def plain_bench(ctx, torch, cfg, steps, warmup, seed):
         name = f"idx2g::k_encode_{kind} (level 0)" if kind else "idx2g::k_encode_level (level 0)"
         dom_ms, dom_bytes, launches = dom_e, float(np.mean([e["dominant_bytes"] for e in enc])), 1
     else:
-        name = (f"idx2g::k_walk_b32 + idx2g::k_decode_{kind} (level 0)" if kind
+        name = (f"idx2g::k_walk + idx2g::k_decode_{kind} (level 0)" if kind
                 else "idx2g::k_decode_level (level 0)")
         dom_ms, dom_bytes = dom_d, float(np.mean([d["dominant_bytes"] for d in dec]))
         launches = int(np.mean([d["dominant_launches"] for d in dec]))

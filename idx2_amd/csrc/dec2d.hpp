@@ -1,6 +1,6 @@
 // Decode of full 2D 32x32 bricks, 32 per workgroup iteration (DecodeBrick /
 // DecodeSubband, Source/Core/idx2Decode.cpp:85-321, for 2D bricks): the
-// stream walks run in k_walk_b32 as for 3D bricks; here the plane words of
+// stream walks run in k_walk (2D shape); here the plane words of
 // up to kGroup bricks' same subband are turned back into values in one pass
 // (a 2D subband has at most 25 blocks) into an LDS array [brick][j][i], from
 // which every brick slot takes its samples straight into its register

@@ -166,53 +166,77 @@ __device__ __forceinline__ void unpark_odd(double (&c)[kN], const double* Z1, co
   for (int k = 0; k < 16; ++k) c[2 * k + 1] = q[k * 256];
 }
 
-// Steps 1-2, one 256-thread workgroup per brick (k_walk_b32): stream
-// staging and the pipelined walks.  Wave w walks subbands 2w and 2w + 1, one
-// per 32-lane segment: lane l of a segment owns key g0 - l of the current
-// group of 32 keys.
+// Steps 1-2, one workgroup per brick (k_walk): stream staging and the
+// pipelined walks.  Wave w walks subbands 2w and 2w + 1, one per 32-lane
+// segment: lane l of a segment owns key g0 - l of the current group of 32
+// keys.  The walk's shape is a template: 3D bricks (8 subbands of <= 128
+// blocks, 256 threads) and 2D bricks (4 subbands of <= 32 blocks, 128
+// threads, a small footprint so that many workgroups stay resident).
 constexpr int kWalkThreads = 256;
 constexpr int kWalkSlots = 64;                // staged streams per subband
-constexpr int kWalkStreamBytes = 20 * 1024;   // LDS stream buffer per brick
+// LDS stream buffer per brick.  Bricks whose streams do not fit walk them
+// from HBM at nearly the same speed, and a small buffer leaves room for six
+// resident workgroups per CU (measured on the 1024^3 float32 decode: 4 KB
+// 24.4 ms, 20 KB 25.5 ms, 44 KB 29.3 ms).
+constexpr int kWalkStreamBytes = 4 * 1024;
 
+template <int NSB, int MB, int TH, int SBYTES>
+struct WalkShape {
+  static constexpr int nsb = NSB;        // subbands
+  static constexpr int mb = MB;          // blocks per subband (power of two)
+  static constexpr int th = TH;          // threads: two subbands per wave
+  static constexpr int sbytes = SBYTES;  // LDS stream buffer
+  static_assert(2 * (TH / 64) == NSB, "one wave per two subbands");
+};
+using Walk3D = WalkShape<8, 128, kWalkThreads, kWalkStreamBytes>;
+using Walk2D = WalkShape<4, 32, 128, 2048>;
+
+template <class WS>
 struct WalkLds {
-  uint32_t sbuf[kWalkStreamBytes / 4 + 8];
-  u64 soff[8][kWalkSlots];
-  uint32_t sstart[8][kWalkSlots];
-  uint32_t ssize[8][kWalkSlots];
-  short E[8][128];
-  unsigned char nbps[8][128];
-  unsigned char nsave[8][128];
-  unsigned char ssave[8][128];
-  uint32_t wsum[kWalkThreads / 64];
-  int klo[8], khi[8];
+  uint32_t sbuf[WS::sbytes / 4 + 8];
+  u64 soff[WS::nsb][kWalkSlots];
+  uint32_t sstart[WS::nsb][kWalkSlots];
+  uint32_t ssize[WS::nsb][kWalkSlots];
+  short E[WS::nsb][WS::mb];
+  unsigned char nbps[WS::nsb][WS::mb];
+  unsigned char nsave[WS::nsb][WS::mb];
+  unsigned char ssave[WS::nsb][WS::mb];
+  uint32_t wsum[WS::th / 64];
+  int klo[WS::nsb], khi[WS::nsb];
   int total, staged;
 };
 
 // Exponent of block b of subband sb (EncodeBrickSubbandExponents layout),
 // kSkip for blocks that are not coded.
-__device__ __forceinline__ int block_exponent(const DecArgs& a, const uint32_t* s_blocks, int sb, int b, int dense) {
-  const uint32_t bw = s_blocks[sb * 128 + b];
+__device__ __forceinline__ int block_exponent(const DecArgs& a, const uint32_t* s_blocks, int sb, int b, int dense,
+                                              int stride = 128) {
+  const uint32_t bw = s_blocks[sb * stride + b];
   if ((bw >> 24) & 1) return kSkip;
   const int ci = int(bw >> 25);
   const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (a.k.dtype ? 2 : 1);
   return a.k.dtype ? int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023 : int(a.exps[row]) - 127;
 }
 
-__device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, const uint32_t* s_blocks) {
+// s_blocks: the level's block tables, [subband][WS::mb].  Plane words and
+// counts go to the batch arrays in the 3D layout ([brick][8][128][npl] and
+// [brick][8][128]) whatever the shape.
+template <class WS>
+__device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi, const uint32_t* s_blocks) {
+  constexpr int NSB = WS::nsb, MB = WS::mb, TH = WS::th;
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const u64 lin = a.lin[bi];
   const int dense = a.dense[bi];
-  if (tid < 8) { W.klo[tid] = 0x7fffffff; W.khi[tid] = -0x7fffffff; }
+  if (tid < NSB) { W.klo[tid] = 0x7fffffff; W.khi[tid] = -0x7fffffff; }
   __syncthreads();
-  for (int i = tid; i < 8 * 128; i += kWalkThreads) {
-    const int sb = i >> 7, b = i & 127;
+  for (int i = tid; i < NSB * MB; i += TH) {
+    const int sb = i / MB, b = i % MB;
     W.nbps[sb][b] = 0;
     W.nsave[sb][b] = 0;
     W.ssave[sb][b] = 0;
     if (sb >= k.nsb || b >= a.blk_n[sb]) continue;
-    const int e = block_exponent(a, s_blocks, sb, b, dense);
+    const int e = block_exponent(a, s_blocks, sb, b, dense, MB);
     W.E[sb][b] = (short)e;
     if (e == kSkip) continue;
     const int stop = max(64 - k.endbp, 57 + a.etol - e);
@@ -225,7 +249,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, co
   // directory entries of the first kWalkSlots keys of each subband, regions in LDS
   {
     uint32_t base = 0;
-    for (int i0 = 0; i0 < 8 * kWalkSlots; i0 += kWalkThreads) {
+    for (int i0 = 0; i0 < NSB * kWalkSlots; i0 += TH) {
       const int i = i0 + tid, sb = i / kWalkSlots, kk = i % kWalkSlots;
       uint32_t sz = 0;
       u64 off = 0;
@@ -245,21 +269,21 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, co
       uint32_t wb = base;
       for (int j = 0; j < w; ++j) wb += W.wsum[j];
       W.sstart[sb][kk] = wb + incl - reg + (uint32_t)(off & 3);
-      for (int j = 0; j < kWalkThreads / 64; ++j) base += W.wsum[j];
+      for (int j = 0; j < TH / 64; ++j) base += W.wsum[j];
       __syncthreads();
     }
     if (tid == 0) {
       bool wide = false;
       for (int s = 0; s < k.nsb; ++s) wide |= W.khi[s] - W.klo[s] + 1 > kWalkSlots;
       W.total = (int)base;
-      W.staged = !wide && base <= (uint32_t)kWalkStreamBytes;
+      W.staged = !wide && base <= (uint32_t)WS::sbytes;
       atomicAdd(&a.counters[1], (unsigned long long)base);  // stream bytes this query reads
     }
   }
   __syncthreads();
   const bool staged = W.staged;
   if (staged) {
-    for (int i = tid; i < 8 * kWalkSlots; i += kWalkThreads) {
+    for (int i = tid; i < NSB * kWalkSlots; i += TH) {
       const int sb = i / kWalkSlots, kk = i % kWalkSlots;
       const uint32_t sz = W.ssize[sb][kk];
       if (!sz) continue;
@@ -286,10 +310,10 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, co
   const bool sb_ok = sb < k.nsb && ((a.mask >> sb) & 1) && W.khi[sb] >= W.klo[sb];
   const int klo = sb_ok ? W.klo[sb] : 0, khi = sb_ok ? W.khi[sb] : -1;
   const int nblk = sb_ok ? a.blk_n[sb] : 0;
-  const int sbi = sb < 8 ? sb : 0;
+  const int sbi = sb < NSB ? sb : 0;
   const int ngr = sb_ok ? (khi - klo) / 32 + 1 : 0;
   const int ng = max(__builtin_amdgcn_readlane(ngr, 0), __builtin_amdgcn_readlane(ngr, 32));
-  const uint32_t* blocks = s_blocks + sbi * 128;
+  const uint32_t* blocks = s_blocks + sbi * MB;
   u64* xs = a.xs + ((size_t)bi * 8 + sbi) * 128 * npl;
   for (int gi = 0; gi < ng; ++gi) {
     const int g0 = khi - 32 * gi;
@@ -354,7 +378,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds& W, int bi, co
     }
   }
   __syncthreads();
-  for (int i = tid; i < 8 * 128; i += kWalkThreads) a.nbps[(size_t)bi * 1024 + i] = (&W.nbps[0][0])[i];
+  for (int i = tid; i < NSB * MB; i += TH) a.nbps[(size_t)bi * 1024 + (i / MB) * 128 + i % MB] = (&W.nbps[0][0])[i];
 }
 
 // Step 3 for subband sb: plane words -> values in D.v.V (zeroed first; the

@@ -272,12 +272,14 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
   }
 }
 
-// Stream walks of full 32^3 bricks (dec32.hpp): one 256-thread workgroup per
-// brick, plane words and counts to the batch arrays a.xs / a.nbps.
-__global__ void __launch_bounds__(b32::kWalkThreads) k_walk_b32(const DecArgs* __restrict__ table,
-                                                               const int* __restrict__ start, int njobs) {
-  __shared__ b32::WalkLds W;
-  __shared__ uint32_t s_blocks[8 * 128];
+// Stream walks of full bricks (dec32.hpp), one workgroup per brick of shape
+// WS (b32::Walk3D for 32^3 bricks, b32::Walk2D for 32x32 bricks); plane words
+// and counts to the batch arrays a.xs / a.nbps.
+template <class WS>
+__global__ void __launch_bounds__(WS::th) k_walk(const DecArgs* __restrict__ table, const int* __restrict__ start,
+                                                 int njobs) {
+  __shared__ b32::WalkLds<WS> W;
+  __shared__ uint32_t s_blocks[WS::nsb * WS::mb];
   const int total = start[njobs];
   int cj = -1;
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
@@ -285,14 +287,14 @@ __global__ void __launch_bounds__(b32::kWalkThreads) k_walk_b32(const DecArgs* _
     const DecArgs& a = table[j];
     if (j != cj) {
       __syncthreads();
-      for (int i = threadIdx.x; i < 8 * 128; i += blockDim.x) {
-        const int s = i >> 7, b = i & 127;
+      for (int i = threadIdx.x; i < WS::nsb * WS::mb; i += blockDim.x) {
+        const int s = i / WS::mb, b = i % WS::mb;
         s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
       }
       cj = j;
       __syncthreads();
     }
-    b32::walk_wg(a, W, g - start[j], s_blocks);
+    b32::walk_wg<WS>(a, W, g - start[j], s_blocks);
     __syncthreads();
   }
 }
@@ -423,7 +425,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
 
 // Full 32x32 bricks of a 2D dataset (dec2d.hpp): 32 bricks per workgroup
 // iteration (brick slot = half-wave), start[] in iterations; the walk
-// (k_walk_b32) has left the plane words in a.xs / a.nbps by piece-local
+// (k_walk) has left the plane words in a.xs / a.nbps by piece-local
 // brick index.
 template <int DT>
 __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restrict__ table,
