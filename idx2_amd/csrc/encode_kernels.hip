@@ -722,17 +722,32 @@ __global__ void k_scatter(const uint8_t* __restrict__ slab, const ScatterSeg* __
 // Byte copy by a whole wave: aligned 32-bit destination words, each lane
 // assembling its word from two aligned source words (sources are padded by
 // >= 4 bytes).
-__device__ inline void copy_bytes_wave(uint8_t* d, const uint8_t* s, uint32_t n, int lane) {
+// n bytes from s to d by the G lanes of a group (gl: lane in the group):
+// aligned destination words, source words realigned with alignbyte (the
+// arena is padded, so reading one word past a stream stays inside it).
+template <int G>
+__device__ inline void copy_bytes_group(uint8_t* d, const uint8_t* s, uint32_t n, int gl) {
   const uint32_t head = min(n, (4u - (uint32_t)((uintptr_t)d & 3)) & 3u);
-  if ((uint32_t)lane < head) d[lane] = s[lane];
+  if ((uint32_t)gl < head) d[gl] = s[gl];
   const uint32_t nw = (n - head) >> 2;
   uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
   const uint8_t* sp = s + head;
   const uint32_t sh = (uint32_t)((uintptr_t)sp & 3);
   const uint32_t* sw = reinterpret_cast<const uint32_t*>(sp - sh);
-  for (uint32_t k = lane; k < nw; k += 64) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
+  uint32_t k = gl;
+  for (; k + 3 * G < nw; k += 4 * G) {  // four words in flight per lane
+    uint32_t a[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = sw[k + i * G];
+    uint32_t b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = sw[k + i * G + 1];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dw[k + i * G] = __builtin_amdgcn_alignbyte(b[i], a[i], sh);
+  }
+  for (; k < nw; k += G) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
   const uint32_t done = head + nw * 4;
-  if ((uint32_t)lane < n - done) d[done + lane] = s[done + lane];
+  if ((uint32_t)gl < n - done) d[done + gl] = s[done + gl];
 }
 
 // One workgroup per non-empty chunk (in blob order): write the header, the
@@ -780,10 +795,10 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
     out[vn + j] = byte;
   }
   // total varbyte bytes of the sizes section
-  __shared__ unsigned long long s_vbtotal, s_vboff, s_doff;
-  __shared__ uint32_t s1[256], s2[256];
+  __shared__ unsigned long long s_vbtotal;
+  __shared__ uint32_t s1[4], s2[4];
 
-  if (tid == 0) { s_vbtotal = 0; s_vboff = 0; s_doff = 0; }
+  if (tid == 0) s_vbtotal = 0;
   __syncthreads();
   unsigned long long vbl = 0;
   for (u64 b = b0 + tid; b < b1; b += blockDim.x) {
@@ -794,6 +809,7 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
   __syncthreads();
   const u64 sizes_off = (u64)vn + dbytes;
   const u64 payload_start = sizes_off + s_vbtotal;
+  u64 vbase = 0, dbase = 0;  // sizes / payload bytes of the earlier steps
   for (u64 t0 = b0; t0 < b1; t0 += blockDim.x) {
     const u64 b = t0 + tid;
     uint32_t sz = 0;
@@ -803,19 +819,22 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
       if (e) { sz = entries[e - 1].size; src = entries[e - 1].off; }
     }
     const uint32_t v1 = sz ? (uint32_t)varbyte_len(sz) : 0u;
-    s1[tid] = v1;
-    s2[tid] = sz;
+    // exclusive prefix sums over the step's 256 bricks: wave scans + wave totals
+    const uint32_t i1 = b32::wave_scan_add(v1, lane), i2 = b32::wave_scan_add(sz, lane);
+    if (lane == 63) { s1[wave] = i1; s2[wave] = i2; }
     __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      uint32_t a1 = 0, a2 = 0;
-      if (tid >= o) { a1 = s1[tid - o]; a2 = s2[tid - o]; }
-      __syncthreads();
-      s1[tid] += a1;
-      s2[tid] += a2;
-      __syncthreads();
+    uint32_t w1 = 0, w2 = 0, t1 = 0, t2 = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      if (w < wave) { w1 += s1[w]; w2 += s2[w]; }
+      t1 += s1[w];
+      t2 += s2[w];
     }
-    const u64 vboff = s_vboff + s1[tid] - v1;
-    const u64 doff = s_doff + s2[tid] - sz;
+    __syncthreads();  // s1 / s2 are rewritten by the next step
+    const u64 vboff = vbase + w1 + i1 - v1;
+    const u64 doff = dbase + w2 + i2 - sz;
+    vbase += t1;
+    dbase += t2;
     const u64 dst = c.off + payload_start + doff;
     if (sz) {
       uint8_t* vp = out + sizes_off + vboff;
@@ -824,20 +843,19 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
       do { uint8_t byte = v & 0x7f; v >>= 7; vp[i++] = byte | (v ? 0x80 : 0); } while (v);
       drow[b] = dir_entry(dst, sz);
     }
-    // payloads: the wave copies its 64 lanes' streams one after another,
-    // every lane on each stream (coalesced words)
-    for (int j = 0; j < 64; ++j) {
-      const uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)sz, j);
-      if (!n) continue;
-      const u64 d = ((u64)(uint32_t)__builtin_amdgcn_readlane((int)(dst >> 32), j) << 32) |
-                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)dst, j);
-      const u64 sr = ((u64)(uint32_t)__builtin_amdgcn_readlane((int)(src >> 32), j) << 32) |
-                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)src, j);
-      copy_bytes_wave(blob + d, arena + sr, n, lane);
+    // payloads: eight lanes per stream, eight of the wave's 64 streams at a
+    // time (the streams average ~100 bytes, so one stream per wave would
+    // leave the copy latency-bound)
+    const int grp = lane >> 3, gl = lane & 7;
+    for (int j0 = 0; j0 < 64; j0 += 8) {
+      const int j = j0 + grp;
+      const uint32_t n = (uint32_t)__shfl((int)sz, j, 64);
+      const u64 d = ((u64)(uint32_t)__shfl((int)(uint32_t)(dst >> 32), j, 64) << 32) |
+                    (uint32_t)__shfl((int)(uint32_t)dst, j, 64);
+      const u64 sr = ((u64)(uint32_t)__shfl((int)(uint32_t)(src >> 32), j, 64) << 32) |
+                     (uint32_t)__shfl((int)(uint32_t)src, j, 64);
+      if (n) copy_bytes_group<8>(blob + d, arena + sr, n, gl);
     }
-    __syncthreads();
-    if (tid == 255) { s_vboff += s1[255]; s_doff += s2[255]; }
-    __syncthreads();
   }
 }
 
