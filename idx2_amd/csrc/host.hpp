@@ -37,7 +37,7 @@ struct idx2g_ctx {
   idx2g_stats last{};
   int ncu = 256;
   // timing events
-  hipEvent_t ev[8];
+  hipEvent_t ev[8] = {};
   // pinned staging of batched uploads (ev[6] marks the last transfer out of it)
   uint8_t* h_stage = nullptr;
   size_t stage_cap = 0;

@@ -23,6 +23,11 @@ using namespace idx2g;
 namespace idx2g {
 
 static const size_t kBlobPad = 64;
+// Chunk bytes read from files get a larger zeroed tail: a corrupted stream
+// can make a walk lane read past the stream's end, by at most one plane per
+// block (<= 193 bits: 64 verbatim, 64 positions, 65 group tests) for <= 128
+// blocks plus its 12-byte window, 3.1 KB.
+static const size_t kFileBlobPad = 4096;
 
 template <class T>
 static int dalloc(T** p, size_t n) {
