@@ -71,10 +71,11 @@ template <int S>
 __device__ __forceinline__ uint32_t xlane(uint32_t v, int lane) {
   if (S == 1) return dpp_u<0xB1>(v);                  // quad_perm [1,0,3,2]
   if (S == 2) return dpp_u<0x4E>(v);                  // quad_perm [2,3,0,1]
-  if (S == 4 || S == 8) {
-    uint32_t up = dpp_u<0x110 + S>(v), dn = dpp_u<0x100 + S>(v);  // row_shr:S, row_shl:S
+  if (S == 8) return dpp_u<0x128>(v);               // row_ror:8: lane ^ 8 inside a row
+  if (S == 4) {
+    uint32_t up = dpp_u<0x114>(v), dn = dpp_u<0x104>(v);  // row_shr:4, row_shl:4
     asm volatile("" : "+v"(up), "+v"(dn));
-    return (lane & S) ? up : dn;
+    return (lane & 4) ? up : dn;
   }
   if (S == 16) {
     const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
@@ -132,18 +133,36 @@ __device__ __forceinline__ int seg_suffix_max_excl(int v, int lane, int seg) {
 }
 
 // One butterfly stage of the 64x64 bit transpose: lanes L (bit s of the
-// lane clear) and U = L ^ s exchange the bit columns whose bit s differs.
+// lane clear) and U = L ^ s exchange the bit columns whose bit s differs:
+//   L: (r & M) | (P << s & ~M),  U: (r & ~M) | (P >> s & M)   (P: partner's r)
+// which is a bit select keep ? r : T with T = M ? P >> s : P << s (the
+// partner's word with adjacent s-bit groups swapped) and keep = M (L) or ~M
+// (U).  gfx950's three-input bit operation does each select in one
+// instruction (0xE4: S2 ? S0 : S1); the s = 32 stage is a single
+// permlane32_swap of the two halves, the s = 16 swap a rotate.
 template <int S>
 __device__ __forceinline__ u64 tstage(u64 r, int lane) {
-  constexpr u64 M = S == 32 ? 0x00000000FFFFFFFFull : S == 16 ? 0x0000FFFF0000FFFFull
-                  : S == 8 ? 0x00FF00FF00FF00FFull : S == 4 ? 0x0F0F0F0F0F0F0F0Full
-                  : S == 2 ? 0x3333333333333333ull : 0x5555555555555555ull;
-  const uint32_t plo = xlane<S>((uint32_t)r, lane), phi = xlane<S>((uint32_t)(r >> 32), lane);
-  const u64 P = ((u64)phi << 32) | plo;
-  const bool up = lane & S;
-  const u64 t = up ? (P >> S) : (P << S);
-  const u64 keep = up ? ~M : M;
-  return (r & keep) | (t & ~keep);
+  uint32_t lo = (uint32_t)r, hi = (uint32_t)(r >> 32);
+  if constexpr (S == 32) {
+    const auto p = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+    return ((u64)(uint32_t)p[1] << 32) | (uint32_t)p[0];
+  } else {
+    constexpr uint32_t M = S == 16 ? 0x0000FFFFu : S == 8 ? 0x00FF00FFu : S == 4 ? 0x0F0F0F0Fu
+                         : S == 2 ? 0x33333333u : 0x55555555u;
+    const uint32_t plo = xlane<S>(lo, lane), phi = xlane<S>(hi, lane);
+    uint32_t tlo, thi;
+    if constexpr (S == 16) {
+      tlo = __builtin_amdgcn_alignbit(plo, plo, 16);
+      thi = __builtin_amdgcn_alignbit(phi, phi, 16);
+    } else {
+      tlo = __builtin_amdgcn_bitop3_b32(plo >> S, plo << S, M, 0xE4);
+      thi = __builtin_amdgcn_bitop3_b32(phi >> S, phi << S, M, 0xE4);
+    }
+    const uint32_t keep = (lane & S) ? ~M : M;
+    lo = __builtin_amdgcn_bitop3_b32(lo, tlo, keep, 0xE4);
+    hi = __builtin_amdgcn_bitop3_b32(hi, thi, keep, 0xE4);
+    return ((u64)hi << 32) | lo;
+  }
 }
 
 // Lane j returns sum_i bit_j(r_i) << i (r_i: the value of lane i).
