@@ -87,8 +87,12 @@ static void morton_blocks(const int sbdims[3], bool skip_full, std::vector<Block
       nd += b.dims[d] > 1;
     }
     b.ndims = nd;
+    // full blocks of subband 0 below the coarsest level are coded in the next
+    // level (CodedInNextLevel): they carry no exponent and no bits, so they
+    // are left out of the table and no kernel spends a step on them (the
+    // Morton rank of the blocks that remain keeps their order)
     b.skip = skip_full && b.dims[0] == 4 && b.dims[1] == 4 && b.dims[2] == 4;
-    out->push_back(b);
+    if (!b.skip) out->push_back(b);
   }
 }
 
