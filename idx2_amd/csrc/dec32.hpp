@@ -41,12 +41,34 @@ __device__ __forceinline__ u64 peek64w(const uint32_t* w32, u64 pos) {
   return (lo >> sh) | ((u64)w[2] << (64 - sh));
 }
 
+// 128 bits of one lane's stream kept in registers: every lane of a walk
+// reads a different stream, so each load instruction touches 64 cache lines;
+// a 128-bit refill serves two to three planes instead of two 64-bit windows
+// per plane.
+struct BitBuf {
+  u64 b0 = 0, b1 = 0;
+  u64 base = 1ull << 63;  // stream bit of b0's bit 0 (dword aligned); none yet
+};
+
+// 64 bits of the stream at bit pos (words padded by >= 16 bytes)
+__device__ __forceinline__ u64 bb_peek(const uint32_t* w32, BitBuf& B, u64 pos) {
+  u64 sh = pos - B.base;
+  if (sh > 64) {  // outside the buffer (also pos < base): refill at pos's word
+    const uint32_t* w = w32 + (pos >> 5);
+    B.b0 = (u64)w[0] | ((u64)w[1] << 32);
+    B.b1 = (u64)w[2] | ((u64)w[3] << 32);
+    B.base = pos & ~31ull;
+    sh = pos & 31;
+  }
+  return sh == 0 ? B.b0 : sh == 64 ? B.b1 : (B.b0 >> sh) | (B.b1 << (64 - sh));
+}
+
 // One bit plane (Decode, Zfp.h:988-1059) from 32-bit words: plane word.
 // A 64-bit window is re-read only when the next group test may need bits
 // beyond it.
-__device__ __forceinline__ u64 decode_plane_win(const uint32_t* w32, u64& pos, int& n, int nv) {
+__device__ __forceinline__ u64 decode_plane_win(const uint32_t* w32, BitBuf& B, u64& pos, int& n, int nv) {
   u64 X = 0;
-  u64 w = peek64w(w32, pos);
+  u64 w = bb_peek(w32, B, pos);
   int avail = 64;
   if (n > 0) {
     if (n >= 64) {
@@ -63,7 +85,7 @@ __device__ __forceinline__ u64 decode_plane_win(const uint32_t* w32, u64& pos, i
   while (n < nv) {
     const int lim = nv - 1 - n;  // bits examined before the implicit last one
     if (avail < lim + 1) {
-      w = peek64w(w32, pos);
+      w = bb_peek(w32, B, pos);
       avail = 64;
     }
     const bool sig = w & 1;
@@ -331,6 +353,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
     const uint32_t* w32 = staged ? W.sbuf : reinterpret_cast<const uint32_t*>(a.blob);
     u64 pos = 0;
     if (have) pos = staged ? (u64)W.sstart[sbi][kk] * 8 : dir_offset(dentry) * 8;
+    BitBuf B;
     const int my_steps = gact ? nblk + min(32, g0 - klo + 1) : 0;
     const int nsteps = max(__builtin_amdgcn_readlane(my_steps, 0), __builtin_amdgcn_readlane(my_steps, 32));
     int n_out = 0, stop_out = 0;
@@ -362,7 +385,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
           } else {
             const int nd = (((bwc >> 15) & 7) > 1) + (((bwc >> 18) & 7) > 1) + (((bwc >> 21) & 7) > 1);
             int n = n_in;
-            const u64 X = decode_plane_win(w32, pos, n, 1 << (2 * nd));
+            const u64 X = decode_plane_win(w32, B, pos, n, 1 << (2 * nd));
             n_out = n;
             xs[b * npl + (63 - bp)] = X;
             W.nbps[sbi][b] = (unsigned char)(64 - bp);
