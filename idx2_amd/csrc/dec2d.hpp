@@ -119,7 +119,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
       const int s0a = ux ? 1 : dx;  // first non-unit axis
       const int s1a = (ux && uy) ? dx : 0;
       const int i1 = r & 3, i2 = r >> 2;
-      const bool rowok = live && ((nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
+      const bool rowok = bin && ((nd == 2 && i2 == 0) || (nd <= 1 && r == 0));  // zeros when not decoded
       const int n0 = nd >= 1 ? 4 : 1;
       i64 w4[4];
 #pragma unroll
@@ -129,7 +129,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
       const int vbase = (sg * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1a : 0);
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0)
-        if (rowok && i0 < n0) P[vbase + i0 * s0a] = scale * i64_to_f64(w4[i0]);
+        if (rowok && i0 < n0) P[vbase + i0 * s0a] = live ? scale * i64_to_f64(w4[i0]) : 0.0;
     }
     b32::wave_sync_lds();
   }

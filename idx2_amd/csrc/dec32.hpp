@@ -404,8 +404,9 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
   for (int i = tid; i < NSB * MB; i += TH) a.nbps[(size_t)bi * 1024 + (i / MB) * 128 + i % MB] = (&W.nbps[0][0])[i];
 }
 
-// Step 3 for subband sb: plane words -> values in D.v.V (zeroed first; the
-// parent's interior already copied for subband 0 below the coarsest level).
+// Step 3 for subband sb: plane words -> values in D.v.V, every sample of
+// the table's blocks (the parent's interior is copied for subband 0 below
+// the coarsest level).
 template <int DT>
 __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
                                                const uint32_t* s_blocks, const u64* XS) {
@@ -504,7 +505,10 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       const int s1 = ux ? (uy ? dx : dxy) : (uy ? dxy : 0);
       const int s2 = (ux && uy && uz) ? dxy : 0;
       const int i1 = r & 3, i2 = r >> 2;
-      const bool rowok = live && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
+      // every block of the table writes its samples (zeros when it is not
+      // decoded), so V needs no zero fill (the blocks tile the subband but for
+      // the subband-0 interior, which holds the parent's data)
+      const bool rowok = bin && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
       const int n0 = nd >= 1 ? 4 : 1;
       i64 w4[4];
 #pragma unroll
@@ -514,7 +518,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0)
-        if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = scale * i64_to_f64(w4[i0]);
+        if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = live ? scale * i64_to_f64(w4[i0]) : 0.0;
     }
     wave_sync_lds();
   }

@@ -307,17 +307,18 @@ __device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, con
   const BrickConst& k = a.k;
   const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
   const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
-  for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
-  __syncthreads();
-  if ((a.mask >> sb) & 1) {
+  __syncthreads();  // V free
+  if (!((a.mask >> sb) & 1)) {  // subband not decoded: zeros
+    for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
+  } else {
     if (sb == 0 && !a.coarsest) {
-      // parent's decoded data: subband-0 interior (CopyExtentGrid, Volume.h:1153-1173)
+      // parent's decoded data: subband-0 interior (CopyExtentGrid, Volume.h:1153-1173);
+      // the blocks below write every other sample
       for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
         const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
         D.v.V[(z * 17 + y) * 17 + x] =
             a.pgrid[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
       }
-      __syncthreads();
     }
     b32::inverse_blocks<DT>(a, sb, dx, dy, D, p, s_blocks, XS);
   }
@@ -497,16 +498,17 @@ __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restric
     auto stage = [&](int sb, int s0, int G) {
       const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
       const int dx = 17 - hx, dy = 17 - hy;
-      for (int t = p.tid; t < G * dx * dy; t += b2d::kTh) D.V[t] = 0.0;
-      __syncthreads();
-      if (!((a.mask >> sb) & 1)) return;  // subband not decoded: zeros
-      if (sb == 0 && !a.coarsest) {
+      if (!((a.mask >> sb) & 1)) {  // subband not decoded: zeros
+        for (int t = p.tid; t < G * dx * dy; t += b2d::kTh) D.V[t] = 0.0;
+        __syncthreads();
+        return;
+      }
+      if (sb == 0 && !a.coarsest) {  // the blocks below write every other sample
         for (int t = p.tid; t < G * 256; t += b2d::kTh) {
           const int gg = t >> 8, x = t & 15, y = (t >> 4) & 15;
           D.V[(gg * dy + y) * dx + x] =
               a.pgrid[(size_t)(s_b3[s0 + gg][1] * 16 + y) * a.pgdims[0] + (s_b3[s0 + gg][0] * 16 + x)];
         }
-        __syncthreads();
       }
       b2d::inverse_blocks_2d<DT>(a, sb, s0, G, dx, dy, D, pb, s_blocks + sb * 32, a.blk_n[sb], bi0, D.V);
       __syncthreads();
