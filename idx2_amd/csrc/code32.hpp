@@ -46,9 +46,11 @@ struct CodeLds {
   uint32_t pool[16384];
   short E[128];                   // block exponent (kSkip: not coded)
   unsigned char np[128];          // coded planes of the block
-  uint32_t kbits[kMaxKeys];       // bits of each key stream
-  uint32_t kbyte[kMaxKeys];       // byte offset of each key stream
-  short kfirst[kMaxKeys];
+  // per key stream, indexed by key % kMaxKeys (a subband's keys span fewer
+  // than kMaxKeys): bits, byte offset, first block with a plane in it
+  uint32_t kbits[kMaxKeys];
+  uint32_t kbyte[kMaxKeys];
+  int kfirst[kMaxKeys];
   unsigned char p3[64], p2[16];
   uint16_t e8[256];               // byte -> bits with a 1 inserted after every 1
   int klo, khi;
@@ -337,10 +339,11 @@ __device__ __forceinline__ u64 coef_u(const i64* Tq, const CL& C, int nd, int c)
 
 // Plane mode after the transpose: lane j holds plane word X of block slot
 // `slot` (of the wave's group) and plane pp.  seg: lanes per block (32 for
-// f32 pairs, 64 for f64).
-template <class CL>
+// f32 pairs, 64 for f64).  kAcc: also add the plane's length to its key
+// stream's total and lower the stream's first block (e: block exponent).
+template <bool kAcc = false, class CL>
 __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bool blk_ok, int np, int nd, int pp,
-                                           CL& C, u64* XG, int npl) {
+                                           CL& C, u64* XG, int npl, int e = 0) {
   const int k = lane & (seg - 1);
   const bool coded = blk_ok && pp < np;
   const int t = (coded && X) ? 64 - clz64(X) : 0;
@@ -351,6 +354,11 @@ __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bo
     const int len = plane_len_cf(X, n, 1 << (2 * nd));
     lno_at(C, npl, blk, pp) = (uint32_t)len | ((uint32_t)n_in << 8);
     *plane_at(C, XG, npl, blk, pp) = X;
+    if constexpr (kAcc) {
+      const int ring = ((63 - pp) + e + kBpBiasDev) & (kMaxKeys - 1);
+      atomicAdd(&C.kbits[ring], (uint32_t)len);
+      atomicMin(&C.kfirst[ring], blk);
+    }
   }
 }
 
@@ -460,6 +468,8 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
     const int np_s[4] = {__builtin_amdgcn_readlane(np, 0), __builtin_amdgcn_readlane(np, 16),
                          __builtin_amdgcn_readlane(np, 32), __builtin_amdgcn_readlane(np, 48)};
+    const int e_s[4] = {__builtin_amdgcn_readlane(emax, 0), __builtin_amdgcn_readlane(emax, 16),
+                        __builtin_amdgcn_readlane(emax, 32), __builtin_amdgcn_readlane(emax, 48)};
     if (DT == 0) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
@@ -471,7 +481,8 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
         const int blk = g * 4 + slot;
         const int nds = (lane >> 5) ? nd_s[2 * pr + 1] : nd_s[2 * pr];
         const int nps = (lane >> 5) ? np_s[2 * pr + 1] : np_s[2 * pr];
-        plane_mode(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG, 32);
+        const int es = (lane >> 5) ? e_s[2 * pr + 1] : e_s[2 * pr];
+        plane_mode<!kSeg>(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG, 32, es);
       }
     } else {
 #pragma unroll
@@ -479,7 +490,7 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
         const u64 u = coef_u(T + qq * 64, C, nd_s[qq], lane);
         const u64 X = bit_transpose(u, lane);
         const int blk = g * 4 + qq;
-        plane_mode(X, lane, 64, blk, blk < nblk, np_s[qq], nd_s[qq], 63 - lane, C, XG, 64);
+        plane_mode<!kSeg>(X, lane, 64, blk, blk < nblk, np_s[qq], nd_s[qq], 63 - lane, C, XG, 64, e_s[qq]);
       }
     }
     wave_sync_lds();
@@ -503,6 +514,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   if (kOdd) stage_subband_odd(Z1, S, C.v.V, p, hx, hy);
   else stage_subband_even(c, S, C.v.V, p, hx, hy);
   if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; C.over = 0; }
+  if (p.tid < kMaxKeys) { C.kbits[p.tid] = 0; C.kfirst[p.tid] = 0x7fffffff; }
   __syncthreads();
   block_stage<DT>(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
   __syncthreads();
@@ -517,36 +529,17 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     __syncthreads();
     return;
   }
-  for (int kk = p.w; kk < nk; kk += kTh / 64) {
-    const int key = klo + kk;
-    uint32_t carry = 0;
-    int first = -1;
-    for (int c0 = 0; c0 < nblk; c0 += 64) {
-      const int b = c0 + lane;
-      const int e = b < nblk ? C.E[b] : kSkip;
-      const int pp = 63 - (key - kBpBiasDev - e);
-      const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
-      const uint32_t len = coded ? (lno_at(C, npl, b, pp) & 0xff) : 0u;
-      const uint32_t incl = wave_scan_add(len, lane);
-      if (coded) lno_at(C, npl, b, pp) |= (incl - len + carry) << 15;
-      const u64 m = __ballot(coded);
-      if (first < 0 && m) first = c0 + __ffsll((long long)m) - 1;
-      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-      }
-      if (lane == 0) {
-        C.kbits[kk] = carry;
-      C.kfirst[kk] = (short)first;
-    }
-  }
-  __syncthreads();
-  // byte layout of the key streams + arena reservation (wave 0)
+  // the block stage has summed each key stream's bits; wave 0 lays out the
+  // streams and reserves their arena slice while the other waves compute each
+  // (block, plane)'s bit offset inside its stream
+  constexpr int M = kMaxKeys - 1;
   if (p.w == 0) {
     uint32_t carry = 0;
     for (int c0 = 0; c0 < nk; c0 += 64) {
       const int kk = c0 + lane;
-      const uint32_t by = kk < nk ? (C.kbits[kk] + 7) >> 3 : 0u;
+      const uint32_t by = kk < nk ? (C.kbits[(klo + kk) & M] + 7) >> 3 : 0u;
       const uint32_t incl = wave_scan_add(by, lane);
-      if (kk < nk) C.kbyte[kk] = incl - by + carry;
+      if (kk < nk) C.kbyte[(klo + kk) & M] = incl - by + carry;
       carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     }
     if (lane == 0) {
@@ -579,6 +572,21 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
       h.ebase = ebs;
       a.hdrs[(size_t)dense * k.nsb + sb] = h;
     }
+  } else {
+    for (int kk = p.w - 1; kk < nk; kk += kTh / 64 - 1) {
+      const int key = klo + kk;
+      uint32_t carry = 0;
+      for (int c0 = 0; c0 < nblk; c0 += 64) {
+        const int b = c0 + lane;
+        const int e = b < nblk ? C.E[b] : kSkip;
+        const int pp = 63 - (key - kBpBiasDev - e);
+        const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
+        const uint32_t len = coded ? (lno_at(C, npl, b, pp) & 0xff) : 0u;
+        const uint32_t incl = wave_scan_add(len, lane);
+        if (coded) lno_at(C, npl, b, pp) |= (incl - len + carry) << 15;
+        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+      }
+    }
   }
   __syncthreads();
   if (C.over) return;
@@ -587,10 +595,11 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   const bool staged = nwords <= (uint32_t)kStreamWords;
   uint32_t* gwords = (uint32_t*)(a.arena + base);
   for (int kk = p.tid; kk < nk; kk += kTh) {
+    const int ring = (klo + kk) & M;
     SbEntry e;
-    e.off = base + C.kbyte[kk];
-    e.size = (C.kbits[kk] + 7) >> 3;
-    e.first_block = (uint16_t)(C.kfirst[kk] < 0 ? 0xffff : C.kfirst[kk]);
+    e.off = base + C.kbyte[ring];
+    e.size = (C.kbits[ring] + 7) >> 3;
+    e.first_block = (uint16_t)(C.kfirst[ring] == 0x7fffffff ? 0xffff : C.kfirst[ring]);
     e.key = (uint16_t)(klo + kk);
     a.entries[C.ebase + kk] = e;
   }
@@ -608,8 +617,8 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     const uint32_t lno = lno_at(C, npl, b, pp);
     u64 lo, hi;
     plane_bits(*plane_at(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
-    const int kk = (63 - pp) + e + kBpBiasDev - klo;
-    const u64 pos = (u64)C.kbyte[kk] * 8 + (lno >> 15);
+    const int ring = ((63 - pp) + e + kBpBiasDev) & M;
+    const u64 pos = (u64)C.kbyte[ring] * 8 + (lno >> 15);
     if (staged) or_bits128<true>(words, pos, lo, hi);
     else or_bits128<false>(words, pos, lo, hi);
   }
