@@ -188,10 +188,10 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
     __threadfence();
   }
   __syncthreads();
-  for (int t = p.tid; t < nblk * npl; t += kTh) {
-    const int b = t / npl, pp = t - b * npl;
+  const int per_w = (nblk + kTh / 64 - 1) / (kTh / 64);  // blocks per wave, coded planes packed over lanes
+  const int wb0 = min(nblk, p.w * per_w);
+  b32::for_coded_planes(C, wb0, min(per_w, nblk - wb0), lane, [&](int b, int pp) {
     const int e = C.E[b];
-    if (e == kSkip || pp >= C.np[b]) continue;
     const int g = b / nb1;
     const uint32_t bw = C.gblk[b];
     const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
@@ -202,7 +202,7 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
     const u64 pos = (u64)C.kbyte[g][kk] * 8 + (lno >> 15);
     if (staged) b32::or_bits128<true>(C.v.stream + C.woff_s[g], pos, lo, hi);
     else b32::or_bits128<false>((uint32_t*)(a.arena + C.base_s[g]), pos, lo, hi);
-  }
+  });
   __syncthreads();
   if (staged)
     for (int g = 0; g < G; ++g) {

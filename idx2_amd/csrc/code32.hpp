@@ -254,6 +254,26 @@ __device__ __forceinline__ void or_bits128(uint32_t* words, u64 pos, u64 lo, u64
     if (d[i]) atomicOr(&w[i], d[i]);
 }
 
+// The coded (block, plane) pairs of blocks [b0, b0 + nb) (nb <= 16), packed
+// over the wave's lanes: a block's coded planes are pp = 0 .. np - 1, so the
+// pairs are numbered by an exclusive scan of np and item i belongs to the
+// block whose range holds it.  f(b, pp) runs once per coded pair.
+template <class CL, class F>
+__device__ __forceinline__ void for_coded_planes(const CL& C, int b0, int nb, int lane, F&& f) {
+  const int bl = b0 + lane;
+  const int np = (lane < nb && C.E[bl] != kSkip) ? (int)C.np[bl] : 0;
+  const uint32_t inc = wave_scan_add((uint32_t)np, lane);
+  const int excl = (int)inc - np;
+  const int tot = __builtin_amdgcn_readlane((int)inc, 63);
+  for (int i0 = 0; i0 < tot; i0 += 64) {
+    const int it = i0 + lane;
+    int j = 0;  // blocks that end at or before item it
+    for (int q = 0; q < nb; ++q) j += __builtin_amdgcn_readlane((int)inc, q) <= it;
+    const int start = __shfl(excl, j < nb ? j : 0, 64);
+    if (it < tot) f(b0 + j, it - start);
+  }
+}
+
 __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -608,10 +628,12 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
   if (!staged) __threadfence();
   __syncthreads();
-  for (int t = p.tid; t < nblk * npl; t += kTh) {
-    const int b = t / npl, pp = t - b * npl;
+  // each wave emits the coded planes of its share of the blocks, packed
+  // over its lanes (no lane idles on a plane beyond a block's last)
+  const int per_w = (nblk + kTh / 64 - 1) / (kTh / 64);
+  const int wb0 = min(nblk, p.w * per_w);
+  for_coded_planes(C, wb0, min(per_w, nblk - wb0), lane, [&](int b, int pp) {
     const int e = C.E[b];
-    if (e == kSkip || pp >= C.np[b]) continue;
     const uint32_t bw = blocks[b];
     const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
     const uint32_t lno = lno_at(C, npl, b, pp);
@@ -621,7 +643,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     const u64 pos = (u64)C.kbyte[ring] * 8 + (lno >> 15);
     if (staged) or_bits128<true>(words, pos, lo, hi);
     else or_bits128<false>(words, pos, lo, hi);
-  }
+  });
   __syncthreads();
   if (staged)
     for (uint32_t i = p.tid; i < nwords; i += kTh) gwords[i] = words[i];
