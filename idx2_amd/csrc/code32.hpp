@@ -490,7 +490,10 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
                          __builtin_amdgcn_readlane(np, 32), __builtin_amdgcn_readlane(np, 48)};
     const int e_s[4] = {__builtin_amdgcn_readlane(emax, 0), __builtin_amdgcn_readlane(emax, 16),
                         __builtin_amdgcn_readlane(emax, 32), __builtin_amdgcn_readlane(emax, 48)};
-    if (DT == 0) {
+    // f32, and f64 rounds whose blocks code at most 32 planes: only the top
+    // 32 bits of the coefficients are coded, two blocks share a transpose
+    const bool narrow = DT == 0 || max(max(np_s[0], np_s[1]), max(np_s[2], np_s[3])) <= 32;
+    if (narrow) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
         const u64 uA = coef_u(T + (2 * pr) * 64, C, nd_s[2 * pr], lane);
@@ -502,7 +505,7 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
         const int nds = (lane >> 5) ? nd_s[2 * pr + 1] : nd_s[2 * pr];
         const int nps = (lane >> 5) ? np_s[2 * pr + 1] : np_s[2 * pr];
         const int es = (lane >> 5) ? e_s[2 * pr + 1] : e_s[2 * pr];
-        plane_mode<!kSeg>(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG, 32, es);
+        plane_mode<!kSeg>(X, lane, 32, blk, blk < nblk, nps, nds, 31 - (lane & 31), C, XG, DT ? 64 : 32, es);
       }
     } else {
 #pragma unroll

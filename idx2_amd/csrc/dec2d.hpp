@@ -67,7 +67,12 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
     }
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
-    if (DT == 0) {
+    // f32, and f64 rounds whose blocks have at most 32 planes decoded: two
+    // blocks share a transpose of their top 32 planes
+    const int nbq = bin ? nbps_of(b) : 0;
+    const bool narrow = DT == 0 || max(max(__builtin_amdgcn_readlane(nbq, 0), __builtin_amdgcn_readlane(nbq, 16)),
+                                       max(__builtin_amdgcn_readlane(nbq, 32), __builtin_amdgcn_readlane(nbq, 48))) <= 32;
+    if (narrow) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
         const int slot = 2 * pr + (lane >> 5);

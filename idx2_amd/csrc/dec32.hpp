@@ -438,14 +438,19 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
     }
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
-    // plane words -> coefficients (bit transpose), inverse order, negabinary
-    if (DT == 0) {
+    // plane words -> coefficients (bit transpose), inverse order, negabinary;
+    // f32, and f64 rounds whose blocks have at most 32 planes decoded: two
+    // blocks share a transpose of their top 32 planes
+    const int nbq = bin ? (int)D.nbps[sb][b] : 0;
+    const bool narrow = DT == 0 || max(max(__builtin_amdgcn_readlane(nbq, 0), __builtin_amdgcn_readlane(nbq, 16)),
+                                       max(__builtin_amdgcn_readlane(nbq, 32), __builtin_amdgcn_readlane(nbq, 48))) <= 32;
+    if (narrow) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
         const int slot = 2 * pr + (lane >> 5);
         const int blk = g * 4 + slot;
         const int pp = 31 - (lane & 31);
-        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[blk * 32 + pp] : 0ull;
+        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[blk * (DT ? 64 : 32) + pp] : 0ull;
         const u64 R = bit_transpose(X, lane);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
