@@ -14,7 +14,8 @@
 //   * the face lines that are not covered above are lifted one line per wave
 //     with one point per lane.
 // The coarse levels 1-4 of the extrapolation (17^3 and smaller) run on an
-// LDS copy of the even subgrid.
+// LDS copy of the even subgrid: level 1 one line per thread, levels 2-4 by
+// one wave without workgroup barriers.
 //
 // Arithmetic is the reference's (FLiftCdf53/ILiftCdf53, Source/Core/
 // Wavelet.h:177-310; ExtrapolateCdf53 and the subband transform,
@@ -65,6 +66,14 @@ __device__ __forceinline__ double lane_next(double v) {  // value of lane + 1
 // Keep a value materialised here: stops the compiler from sinking a whole
 // unrolled pass into its consumers (which multiplies live registers).
 __device__ __forceinline__ void pin(double& v) { asm volatile("" : "+v"(v)); }
+
+// Orders one wave's LDS accesses (stores before, loads after); the wave's
+// LDS instructions execute in issue order.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ---- one line held one point per lane: lane index i in [0, n) ----
 // kFwd/kInv: D = n odd points.  kExt: D = 32 data points plus the
@@ -352,44 +361,114 @@ __device__ __forceinline__ void zpass(double (&c)[kN], Lds& S, const Pos& p) {
   }
 }
 
-// Lines of the LDS subgrid along axis A with index stride SS and D points
-// (all compile-time, so the line/point split is shifts and multiplies).
-template <int MODE, int A, int SS, int D>
-__device__ __forceinline__ void sub_pass(double* sub, const Pos& p) {
-  constexpr int sa = (A == 0 ? 1 : A == 1 ? 17 : 289) * SS;
-  constexpr int s1 = (A == 0 ? 17 : 1) * SS;
-  constexpr int s2 = (A == 2 ? 17 : 289) * SS;
-  constexpr int per = 64 / D, nl = D * D;
-  const int slot = p.lane / D, i = p.lane - slot * D;
-  for (int j0 = p.w * per; j0 < nl; j0 += (kTh / 64) * per) {
-    const int j = j0 + slot;
-    const bool act = slot < per && j < nl;
-    double* q = sub + (j % D) * s1 + (j / D) * s2 + i * sa;
-    double v = act ? *q : 0.0;
-    v = lift_lanes<MODE>(v, i, D);
-    if (act) *q = v;
+// One line of D points held in registers: the same operations, in the same
+// order, as lift_lanes / zcol_fwd / zcol_inv.
+template <int MODE, int D>
+__device__ __forceinline__ void line_lift(double (&v)[D]) {
+  if (MODE == kFwd) {
+#pragma unroll
+    for (int o = 1; o < D; o += 2) v[o] = v[o] - (v[o - 1] + v[o + 1]) * 0.5;
+#pragma unroll
+    for (int e = 0; e < D; e += 2) {
+      double t = v[e];
+      if (e > 0) t = t + v[e - 1] * 0.25;
+      if (e < D - 1) t = t + v[e + 1] * 0.25;
+      v[e] = t;
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < D; e += 2) {
+      double t = v[e];
+      if (e > 0) t = t - v[e - 1] * 0.25;
+      if (e < D - 1) t = t - v[e + 1] * 0.25;
+      v[e] = t;
+    }
+#pragma unroll
+    for (int o = 1; o < D; o += 2) v[o] = v[o] + (v[o - 1] + v[o + 1]) * 0.5;
   }
 }
 
-// One level of the subgrid transform (level L >= 1: stride 2^(L-1), D points
-// per line): X, Y, Z forward or Z, Y, X inverse.
-template <int MODE, int L>
-__device__ __forceinline__ void sub_level(double* sub, const Pos& p) {
-  constexpr int SS = 1 << (L - 1);
-  constexpr int D = L == 1 ? 17 : L == 2 ? 9 : L == 3 ? 5 : 3;
+// Line t (< D * D) of the subgrid along axis A, lifted in registers.  The
+// forward lift streams along the line (odd point o, then the even point o - 1
+// it completes), so only a few points are live at a time.
+template <int MODE, int A, int SS, int D>
+__device__ __forceinline__ void sub_line(double* sub, int t) {
+  constexpr int sa = (A == 0 ? 1 : A == 1 ? 17 : 289) * SS;
+  constexpr int s1 = (A == 0 ? 17 : 1) * SS;
+  constexpr int s2 = (A == 2 ? 17 : 289) * SS;
+  double* q = sub + (t % D) * s1 + (t / D) * s2;
+  double v[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) v[i] = q[i * sa];
+  line_lift<MODE, D>(v);
+#pragma unroll
+  for (int i = 0; i < D; ++i) q[i * sa] = v[i];
+}
+
+// All D * D lines of one axis by the lanes of one wave.
+template <int MODE, int A, int SS, int D>
+__device__ __forceinline__ void sub_lines_wave(double* sub, int lane) {
+#pragma unroll
+  for (int it = 0; it < (D * D + 63) / 64; ++it)
+    if (lane + 64 * it < D * D) sub_line<MODE, A, SS, D>(sub, lane + 64 * it);
+}
+
+// Levels 2-4 of the subgrid (9^3 points and fewer) forward then inverse, by
+// one wave: 18 passes separated by wave-level ordering only.
+__device__ __forceinline__ void sub_coarse_wave(double* sub, int lane) {
+  sub_lines_wave<kFwd, 0, 2, 9>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 1, 2, 9>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 2, 2, 9>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 0, 4, 5>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 1, 4, 5>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 2, 4, 5>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 0, 8, 3>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 1, 8, 3>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kFwd, 2, 8, 3>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 2, 8, 3>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 1, 8, 3>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 0, 8, 3>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 2, 4, 5>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 1, 4, 5>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 0, 4, 5>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 2, 2, 9>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 1, 2, 9>(sub, lane);
+  wave_lds_order();
+  sub_lines_wave<kInv, 0, 2, 9>(sub, lane);
+}
+
+// Level 1 (17^3 points, 289 lines per axis), one line per thread.
+template <int MODE>
+__device__ __forceinline__ void sub_level1_lines(double* sub, const Pos& p) {
   if (MODE == kFwd) {
-    sub_pass<MODE, 0, SS, D>(sub, p);
+    if (p.tid < 289) sub_line<kFwd, 0, 1, 17>(sub, p.tid);
     __syncthreads();
-    sub_pass<MODE, 1, SS, D>(sub, p);
+    if (p.tid < 289) sub_line<kFwd, 1, 1, 17>(sub, p.tid);
     __syncthreads();
-    sub_pass<MODE, 2, SS, D>(sub, p);
+    if (p.tid < 289) sub_line<kFwd, 2, 1, 17>(sub, p.tid);
     __syncthreads();
   } else {
-    sub_pass<MODE, 2, SS, D>(sub, p);
+    if (p.tid < 289) sub_line<kInv, 2, 1, 17>(sub, p.tid);
     __syncthreads();
-    sub_pass<MODE, 1, SS, D>(sub, p);
+    if (p.tid < 289) sub_line<kInv, 1, 1, 17>(sub, p.tid);
     __syncthreads();
-    sub_pass<MODE, 0, SS, D>(sub, p);
+    if (p.tid < 289) sub_line<kInv, 0, 1, 17>(sub, p.tid);
     __syncthreads();
   }
 }
@@ -429,14 +508,10 @@ __device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& 
   __syncthreads();
   sub_copy<true>(c, S, p);
   __syncthreads();
-  sub_level<kFwd, 1>(S.u.sub, p);
-  sub_level<kFwd, 2>(S.u.sub, p);
-  sub_level<kFwd, 3>(S.u.sub, p);
-  sub_level<kFwd, 4>(S.u.sub, p);
-  sub_level<kInv, 4>(S.u.sub, p);
-  sub_level<kInv, 3>(S.u.sub, p);
-  sub_level<kInv, 2>(S.u.sub, p);
-  sub_level<kInv, 1>(S.u.sub, p);
+  sub_level1_lines<kFwd>(S.u.sub, p);
+  if (p.w == 0) sub_coarse_wave(S.u.sub, p.lane);
+  __syncthreads();
+  sub_level1_lines<kInv>(S.u.sub, p);
   sub_copy<false>(c, S, p);
   __syncthreads();
   zpass<kInv>(c, S, p);
