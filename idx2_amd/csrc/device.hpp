@@ -350,6 +350,10 @@ __device__ __forceinline__ int wave_min_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
   return v;
 }
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+  return v;
+}
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // --------------------------------------------------------------------------

@@ -655,18 +655,34 @@ __global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ j
   const uint32_t* row = J.idx + ((size_t)sb * nk + kk) * J.span;
   unsigned long long cnt = 0, sz = 0, vb = 0;
   u64 first = ~0ull, last = 0;
-  for (u64 b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
-    uint32_t e = row[b];
-    if (!e || entries[e - 1].size == 0) continue;
-    const uint32_t s = entries[e - 1].size;
-    cnt += 1;
-    sz += s;
-    vb += varbyte_len(s);
-    first = min(first, b);
-    last = max(last, b);
-  }
   __shared__ unsigned long long s_cnt, s_sz, s_vb, s_first, s_last;
+  __shared__ uint32_t s_ssz[kMaxChunkSteps], s_svb[kMaxChunkSteps];  // per step of kChunkStep bricks
   if (threadIdx.x == 0) { s_cnt = s_sz = s_vb = 0; s_first = ~0ull; s_last = 0; }
+  for (int i = threadIdx.x; i < J.nsteps; i += blockDim.x) s_ssz[i] = s_svb[i] = 0;
+  __syncthreads();
+  // blockDim.x == kChunkStep: iteration stp covers step stp of the chunk
+  int stp = 0;
+  for (u64 bb = b0; bb < b1; bb += blockDim.x, ++stp) {
+    const u64 b = bb + threadIdx.x;
+    uint32_t s = 0;
+    if (b < b1) {
+      const uint32_t e = row[b];
+      s = e ? entries[e - 1].size : 0u;
+    }
+    const uint32_t v = s ? (uint32_t)varbyte_len(s) : 0u;
+    if (s) {
+      cnt += 1;
+      sz += s;
+      vb += v;
+      first = min(first, b);
+      last = max(last, b);
+    }
+    const uint32_t ws = wave_sum_u(s), wv = wave_sum_u(v);
+    if (lane_id() == 0 && ws) {
+      atomicAdd(&s_ssz[stp], ws);
+      atomicAdd(&s_svb[stp], wv);
+    }
+  }
   __syncthreads();
   if (cnt) {
     atomicAdd(&s_cnt, cnt);
@@ -693,6 +709,15 @@ __global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ j
       c.first_fb = entries[row[s_first] - 1].first_block;
     }
     J.recs[r] = c;
+    // offsets of the steps inside the sizes / payload sections, then totals
+    uint32_t ov = 0, od = 0;
+    uint2* so = J.steps + (size_t)r * (J.nsteps + 1);
+    for (int i = 0; i < J.nsteps; ++i) {
+      so[i] = make_uint2(ov, od);
+      ov += s_svb[i];
+      od += s_ssz[i];
+    }
+    so[J.nsteps] = make_uint2(ov, od);
   }
 }
 
@@ -719,55 +744,31 @@ __global__ void k_scatter(const uint8_t* __restrict__ slab, const ScatterSeg* __
   for (u64 i = nv * 16 + threadIdx.x; i < sg.n; i += blockDim.x) d[i] = s[i];
 }
 
-// Byte copy by a whole wave: aligned 32-bit destination words, each lane
-// assembling its word from two aligned source words (sources are padded by
-// >= 4 bytes).
-// n bytes from s to d by the G lanes of a group (gl: lane in the group):
-// aligned destination words, source words realigned with alignbyte (the
-// arena is padded, so reading one word past a stream stays inside it).
-template <int G>
-__device__ inline void copy_bytes_group(uint8_t* d, const uint8_t* s, uint32_t n, int gl) {
-  const uint32_t head = min(n, (4u - (uint32_t)((uintptr_t)d & 3)) & 3u);
-  if ((uint32_t)gl < head) d[gl] = s[gl];
-  const uint32_t nw = (n - head) >> 2;
-  uint32_t* dw = reinterpret_cast<uint32_t*>(d + head);
-  const uint8_t* sp = s + head;
-  const uint32_t sh = (uint32_t)((uintptr_t)sp & 3);
-  const uint32_t* sw = reinterpret_cast<const uint32_t*>(sp - sh);
-  uint32_t k = gl;
-  for (; k + 3 * G < nw; k += 4 * G) {  // four words in flight per lane
-    uint32_t a[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = sw[k + i * G];
-    uint32_t b[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = sw[k + i * G + 1];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) dw[k + i * G] = __builtin_amdgcn_alignbyte(b[i], a[i], sh);
-  }
-  for (; k < nw; k += G) dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], sh);
-  const uint32_t done = head + nw * 4;
-  if ((uint32_t)gl < n - done) d[done + gl] = s[done + gl];
-}
-
-// One workgroup per non-empty chunk (in blob order): write the header, the
-// varbyte sizes and the payloads, and record where each brick's stream
-// starts (decode directory: blob offset + 1).
+// One workgroup per (non-empty chunk in blob order, step of kChunkStep
+// bricks): the first step writes the chunk's two leading varbytes, every step
+// its share of the presence bitmap, the varbyte sizes and payloads of its
+// bricks, and where each brick's stream starts (decode directory: blob
+// offset + 1).  The steps' offsets inside the sections come from
+// k_chunk_size, so the steps of a chunk run in parallel.
 __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
                                                      int njobs, const SbEntry* __restrict__ entries,
                                                      const uint8_t* __restrict__ arena) {
   const int jb = job_of(start, njobs, blockIdx.x);
   const AsmJob& J = jobs[jb];
-  const int nk = J.nk, klo = J.klo, bpc = J.bpc;
+  const int nk = J.nk, klo = J.klo, bpc = J.bpc, nsteps = J.nsteps;
   const u64 span = J.span;
-  const uint32_t* idx = J.idx;
-  uint8_t* blob = J.blob;
-  u64* dir = J.dir;
-  const ChunkRec c = J.recs[J.order[blockIdx.x - start[jb]]];
-  const size_t rowi = ((size_t)c.sb * nk + (c.key - klo)) * span;
-  const uint32_t* row = idx + rowi;
-  u64* drow = dir + rowi;
+  const int rel = blockIdx.x - start[jb];
+  const int oi = rel / nsteps, step = rel - oi * nsteps;
+  const int ri = J.order[oi];
+  const ChunkRec c = J.recs[ri];
   const u64 b0 = (u64)c.chunk * bpc, b1 = min(b0 + (u64)bpc, span);
+  const u64 s0 = b0 + (u64)step * kChunkStep;
+  if (s0 >= b1) return;
+  const u64 s1e = min(s0 + (u64)kChunkStep, b1);
+  const size_t rowi = ((size_t)c.sb * nk + (c.key - klo)) * span;
+  const uint32_t* row = J.idx + rowi;
+  u64* drow = J.dir + rowi;
+  uint8_t* blob = J.blob;
   uint8_t* out = blob + c.off;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int vn = varbyte_len(c.nbricks), vf = varbyte_len(c.first_brick);
@@ -777,7 +778,7 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
     const uint32_t e = row[b];
     return e && entries[e - 1].size > 0;
   };
-  if (tid == 0) {
+  if (step == 0 && tid == 0) {
     u64 v = c.nbricks;
     int i = 0;
     do { uint8_t byte = v & 0x7f; v >>= 7; out[i++] = byte | (v ? 0x80 : 0); } while (v);
@@ -785,57 +786,53 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
     i = 0;
     do { uint8_t byte = v & 0x7f; v >>= 7; out[vn + i++] = byte | (v ? 0x80 : 0); } while (v);
   }
-  // unary gaps as a presence bitmap, one byte per thread
-  for (u64 j = (u64)vf + tid; j < dbytes; j += blockDim.x) {
-    uint8_t byte = 0;
-    for (int i = 0; i < 8; ++i) {
-      const u64 bit = 8 * (j - vf) + i;
-      if (bit < nbm && present(c.first_brick + 1 + bit)) byte |= uint8_t(1u << i);
+  // unary gaps as a presence bitmap: byte j holds the bits of bricks
+  // first_brick + 1 + 8 (j - vf) + i and belongs to the step that holds the
+  // brick of its first bit
+  {
+    const long long f1 = (long long)c.first_brick + 1;
+    const long long lo = ((long long)s0 - f1 + 7) >> 3, hi = ((long long)s1e - f1 + 7) >> 3;
+    const u64 j0 = (u64)vf + (u64)max(0ll, lo);
+    const u64 j1 = min(dbytes, (u64)vf + (u64)max(0ll, hi));
+    for (u64 j = j0 + tid; j < j1; j += blockDim.x) {
+      uint8_t byte = 0;
+      for (int i = 0; i < 8; ++i) {
+        const u64 bit = 8 * (j - vf) + i;
+        if (bit < nbm && present(c.first_brick + 1 + bit)) byte |= uint8_t(1u << i);
+      }
+      out[vn + j] = byte;
     }
-    out[vn + j] = byte;
   }
-  // total varbyte bytes of the sizes section
-  __shared__ unsigned long long s_vbtotal;
   __shared__ uint32_t s1[4], s2[4];
-
-  if (tid == 0) s_vbtotal = 0;
-  __syncthreads();
-  unsigned long long vbl = 0;
-  for (u64 b = b0 + tid; b < b1; b += blockDim.x) {
-    const uint32_t e = row[b];
-    if (e && entries[e - 1].size) vbl += varbyte_len(entries[e - 1].size);
-  }
-  if (vbl) atomicAdd(&s_vbtotal, vbl);
-  __syncthreads();
+  __shared__ uint32_t s_pre[kChunkStep];
+  __shared__ u64 s_src[kChunkStep];
+  const uint2* so = J.steps + (size_t)ri * (nsteps + 1);
+  const uint2 sto = so[step];
   const u64 sizes_off = (u64)vn + dbytes;
-  const u64 payload_start = sizes_off + s_vbtotal;
-  u64 vbase = 0, dbase = 0;  // sizes / payload bytes of the earlier steps
-  for (u64 t0 = b0; t0 < b1; t0 += blockDim.x) {
-    const u64 b = t0 + tid;
+  const u64 payload_start = sizes_off + so[nsteps].x;
+  {
+    const u64 b = s0 + tid;
     uint32_t sz = 0;
     u64 src = 0;
-    if (b < b1) {
+    if (b < s1e) {
       const uint32_t e = row[b];
       if (e) { sz = entries[e - 1].size; src = entries[e - 1].off; }
     }
     const uint32_t v1 = sz ? (uint32_t)varbyte_len(sz) : 0u;
-    // exclusive prefix sums over the step's 256 bricks: wave scans + wave totals
+    // exclusive prefix sums over the step's bricks: wave scans + wave totals
     const uint32_t i1 = b32::wave_scan_add(v1, lane), i2 = b32::wave_scan_add(sz, lane);
     if (lane == 63) { s1[wave] = i1; s2[wave] = i2; }
     __syncthreads();
-    uint32_t w1 = 0, w2 = 0, t1 = 0, t2 = 0;
+    uint32_t w1 = 0, w2 = 0, t2 = 0;
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       if (w < wave) { w1 += s1[w]; w2 += s2[w]; }
-      t1 += s1[w];
       t2 += s2[w];
     }
-    __syncthreads();  // s1 / s2 are rewritten by the next step
-    const u64 vboff = vbase + w1 + i1 - v1;
-    const u64 doff = dbase + w2 + i2 - sz;
-    vbase += t1;
-    dbase += t2;
-    const u64 dst = c.off + payload_start + doff;
+    const u64 vboff = (u64)sto.x + w1 + i1 - v1;
+    const uint32_t pre = w2 + i2 - sz;  // payload offset inside the step
+    const u64 pbase = c.off + payload_start + sto.y;  // the step's payload in the blob
+    const u64 dst = pbase + pre;
     if (sz) {
       uint8_t* vp = out + sizes_off + vboff;
       u64 v = sz;
@@ -843,18 +840,68 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
       do { uint8_t byte = v & 0x7f; v >>= 7; vp[i++] = byte | (v ? 0x80 : 0); } while (v);
       drow[b] = dir_entry(dst, sz);
     }
-    // payloads: eight lanes per stream, eight of the wave's 64 streams at a
-    // time (the streams average ~100 bytes, so one stream per wave would
-    // leave the copy latency-bound)
-    const int grp = lane >> 3, gl = lane & 7;
-    for (int j0 = 0; j0 < 64; j0 += 8) {
-      const int j = j0 + grp;
-      const uint32_t n = (uint32_t)__shfl((int)sz, j, 64);
-      const u64 d = ((u64)(uint32_t)__shfl((int)(uint32_t)(dst >> 32), j, 64) << 32) |
-                    (uint32_t)__shfl((int)(uint32_t)dst, j, 64);
-      const u64 sr = ((u64)(uint32_t)__shfl((int)(uint32_t)(src >> 32), j, 64) << 32) |
-                     (uint32_t)__shfl((int)(uint32_t)src, j, 64);
-      if (n) copy_bytes_group<8>(blob + d, arena + sr, n, gl);
+    // payloads, destination-driven: each thread fills 16-byte aligned units
+    // of the step's payload range [pbase, pbase + t2).  The stream of a
+    // unit's first byte comes from a binary search over the step's payload
+    // offsets; a unit inside one stream is five aligned source loads and one
+    // 16-byte store, the rest (stream seams, the range's two ends) go byte
+    // by byte.
+    s_pre[tid] = pre;
+    s_src[tid] = src;
+    __syncthreads();
+    const uintptr_t p0 = (uintptr_t)(blob + pbase);
+    for (uintptr_t u = (p0 & ~(uintptr_t)15) + 16 * (uintptr_t)tid; u < p0 + t2; u += 16 * (uintptr_t)blockDim.x) {
+      const long long r0 = (long long)u - (long long)p0;  // negative for a unit that starts before the range
+      const uint32_t rs = r0 < 0 ? 0u : (uint32_t)r0;
+      int j = 0;
+#pragma unroll
+      for (int st = kChunkStep / 2; st; st >>= 1)
+        if (s_pre[j + st] <= rs) j += st;
+      const uint32_t end_j = j < kChunkStep - 1 ? s_pre[j + 1] : t2;
+      const bool whole = r0 >= 0 && (u64)r0 + 16 <= t2;
+      if (whole && (u64)r0 + 16 <= end_j) {
+        const u64 sp = s_src[j] + ((u64)r0 - s_pre[j]);
+        const uint32_t* sw = reinterpret_cast<const uint32_t*>(arena + (sp & ~3ull));
+        const uint32_t sh = (uint32_t)(sp & 3);
+        uint32_t w[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) w[i] = sw[i];
+        uint4 o;
+        o.x = __builtin_amdgcn_alignbyte(w[1], w[0], sh);
+        o.y = __builtin_amdgcn_alignbyte(w[2], w[1], sh);
+        o.z = __builtin_amdgcn_alignbyte(w[3], w[2], sh);
+        o.w = __builtin_amdgcn_alignbyte(w[4], w[3], sh);
+        *reinterpret_cast<uint4*>(u) = o;
+      } else {
+        // source address of every in-range byte first (LDS only), then all
+        // sixteen loads in flight together
+        u64 sa[16];
+        int jj = j;
+#pragma unroll
+        for (int bi = 0; bi < 16; ++bi) {
+          const long long r = r0 + bi;
+          sa[bi] = ~0ull;
+          if (r >= 0 && r < (long long)t2) {
+            while (jj < kChunkStep - 1 && s_pre[jj + 1] <= (uint32_t)r) ++jj;
+            sa[bi] = s_src[jj] + ((u64)r - s_pre[jj]);
+          }
+        }
+        uint8_t v[16];
+#pragma unroll
+        for (int bi = 0; bi < 16; ++bi) v[bi] = sa[bi] != ~0ull ? arena[sa[bi]] : (uint8_t)0;
+        if (whole) {
+          uint32_t o[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            o[i] = (uint32_t)v[4 * i] | ((uint32_t)v[4 * i + 1] << 8) | ((uint32_t)v[4 * i + 2] << 16) |
+                   ((uint32_t)v[4 * i + 3] << 24);
+          *reinterpret_cast<uint4*>(u) = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+#pragma unroll
+          for (int bi = 0; bi < 16; ++bi)
+            if (sa[bi] != ~0ull) reinterpret_cast<uint8_t*>(u)[bi] = v[bi];
+        }
+      }
     }
   }
 }

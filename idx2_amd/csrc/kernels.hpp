@@ -93,13 +93,17 @@ struct AsmJob {
   const u64* lin;            // [nb] linear brick ids
   uint32_t* idx;             // [nsb][nk][span] entry + 1
   ChunkRec* recs;            // [nsb][nk][nchunks]
+  uint2* steps;              // [nsb][nk][nchunks][nsteps + 1]: (sizes, payload) byte offsets of each step, then totals
   const int* order;          // non-empty chunks in blob order
   uint8_t* blob;
   u64* dir;
   int* kr;                   // key range out: [lo, hi]
   u64 span;
-  int nb, nsb, klo, nk, bpc, nchunks;
+  int nb, nsb, klo, nk, bpc, nchunks, nsteps;
 };
+
+constexpr int kChunkStep = 256;                  // bricks per chunk-writer workgroup
+constexpr int kMaxChunkSteps = 32768 / kChunkStep;  // bricks per chunk <= 32768
 
 // Device copies of small host arrays: bytes [off, off + n) of the uploaded
 // slab go to dst.
