@@ -72,38 +72,36 @@ __device__ __forceinline__ void ypass(double (&c)[kN], double* ex, const Pos2& p
   }
 }
 
-// Lines of the first ns slots' LDS subgrids (sub[slot][j * 17 + i] =
-// (2i, 2j)) along axis A with index stride SS and D points per line.
+// Line t of the first ns slots' LDS subgrids (sub[slot][j * 17 + i] =
+// (2i, 2j)) along axis A with index stride SS and D points, lifted in
+// registers by one thread (t < ns * D).
 template <int MODE, int A, int SS, int D>
-__device__ __forceinline__ void sub_pass(double* sub, const Pos2& p, int ns) {
-  constexpr int per = 64 / D;
-  const int nl = ns * D;
+__device__ __forceinline__ void sub_line(double* sub, int t) {
   constexpr int sa = A == 0 ? SS : SS * 17, so = A == 0 ? SS * 17 : SS;
-  const int ls = p.lane / D, i = p.lane - ls * D;
-  for (int j0 = p.w * per; j0 < nl; j0 += (kTh / 64) * per) {
-    const int j = j0 + ls;
-    const bool act = ls < per && j < nl;
-    const int s = j / D, o = j - s * D;
-    double* q = sub + s * kSub + o * so + i * sa;
-    double v = act ? *q : 0.0;
-    v = b32::lift_lanes<MODE>(v, i, D);
-    if (act) *q = v;
-  }
+  const int s = t / D, o = t - s * D;
+  double* q = sub + s * kSub + o * so;
+  double v[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) v[i] = q[i * sa];
+  b32::line_lift<MODE, D>(v);
+#pragma unroll
+  for (int i = 0; i < D; ++i) q[i * sa] = v[i];
 }
 
 template <int MODE, int L>
 __device__ __forceinline__ void sub_level(double* sub, const Pos2& p, int ns) {
   constexpr int SS = 1 << (L - 1);
   constexpr int D = L == 1 ? 17 : L == 2 ? 9 : L == 3 ? 5 : 3;
+  const int nl = ns * D;  // <= 32 * 17 < kTh
   if (MODE == kFwd) {
-    sub_pass<MODE, 0, SS, D>(sub, p, ns);
+    if (p.tid < nl) sub_line<MODE, 0, SS, D>(sub, p.tid);
     __syncthreads();
-    sub_pass<MODE, 1, SS, D>(sub, p, ns);
+    if (p.tid < nl) sub_line<MODE, 1, SS, D>(sub, p.tid);
     __syncthreads();
   } else {
-    sub_pass<MODE, 1, SS, D>(sub, p, ns);
+    if (p.tid < nl) sub_line<MODE, 1, SS, D>(sub, p.tid);
     __syncthreads();
-    sub_pass<MODE, 0, SS, D>(sub, p, ns);
+    if (p.tid < nl) sub_line<MODE, 0, SS, D>(sub, p.tid);
     __syncthreads();
   }
 }
