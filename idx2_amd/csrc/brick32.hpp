@@ -499,7 +499,11 @@ __device__ __forceinline__ void sub_copy(double (&c)[kN], Lds& S, const Pos& p) 
 
 // ExtrapolateCdf53 of a full brick (data extent 32 on every axis): forward
 // levels 0-4 on the growing data grid, then inverse levels 4-0 on 33^3.
-__device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& p) {
+// park: kParkN * kTh doubles of LDS that are free during the subgrid levels;
+// the top kParkN points of every column wait there, which leaves room in
+// the registers for the lines of level 1.
+constexpr int kParkN = 12;
+__device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& p, double* park) {
   xpass<kExt>(c, S, p);
   __syncthreads();
   ypass<kExt>(c, S, p);
@@ -507,11 +511,15 @@ __device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& 
   zpass<kExt>(c, S, p);
   __syncthreads();
   sub_copy<true>(c, S, p);
+#pragma unroll
+  for (int k = 0; k < kParkN; ++k) park[k * kTh + p.tid] = c[kN - kParkN + k];
   __syncthreads();
   sub_level1_lines<kFwd>(S.u.sub, p);
   if (p.w == 0) sub_coarse_wave(S.u.sub, p.lane);
   __syncthreads();
   sub_level1_lines<kInv>(S.u.sub, p);
+#pragma unroll
+  for (int k = 0; k < kParkN; ++k) c[kN - kParkN + k] = park[k * kTh + p.tid];
   sub_copy<false>(c, S, p);
   __syncthreads();
   zpass<kInv>(c, S, p);

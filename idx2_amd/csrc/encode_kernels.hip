@@ -416,7 +416,10 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
       store_minmax(a, sh, dense, mn, mx);
     }
     __syncthreads();
-    b32::extrapolate(c, S, p);
+    // LDS past the subgrid is free until the coding stage (it belongs to CodeLds)
+    static_assert(sizeof(S.u.sub) + 8 + sizeof(double) * b32::kParkN * b32::kTh <= sizeof(b32::CodeLds),
+                  "park area past the subgrid");
+    b32::extrapolate(c, S, p, reinterpret_cast<double*>(&S.u) + (17 * 17 * 17 + 1));
     b32::forward(c, S, p);
     if (!a.coarsest && !(p.x & 1) && !(p.y & 1)) {
       const unsigned pplane = (unsigned)a.pdims[1] * (unsigned)a.pdims[0];
