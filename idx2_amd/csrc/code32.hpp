@@ -323,7 +323,7 @@ __device__ __forceinline__ void stage_subband_even(const double (&c)[kN], const 
 // holds column (2i + hx, 2j + hy) at z = 2k + 1.
 constexpr int kOddOff = 16384;  // doubles into the scratch volume
 __device__ __forceinline__ void park_odd(const double (&c)[kN], double* Z1, const Pos& p) {
-  double* q = Z1 + ((p.x & 1) + 2 * (p.y & 1)) * 16 * 256 + (p.y >> 1) * 16 + (p.x >> 1);
+  auto q = gm(Z1) + ((p.x & 1) + 2 * (p.y & 1)) * 16 * 256 + (p.y >> 1) * 16 + (p.x >> 1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) q[k * 256] = c[2 * k + 1];
 }
@@ -332,7 +332,7 @@ __device__ __forceinline__ void park_odd(const double (&c)[kN], double* Z1, cons
 __device__ __forceinline__ void stage_subband_odd(const double* Z1, const Lds& S, double* V, const Pos& p, int hx,
                                                   int hy) {
   const int dx = 17 - hx, dy = 17 - hy;
-  const double* src = Z1 + (hx + 2 * hy) * 16 * 256;
+  const auto src = gm(Z1) + (hx + 2 * hy) * 16 * 256;
   for (int t = p.tid; t < 16 * 256; t += kTh) {
     const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
     V[(k * dy + j) * dx + i] = src[t];
@@ -467,10 +467,10 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
         const long long row = a.exp_off[sb] + ((long long)bdense * a.ncoded[sb] + ci) * (DT ? 2 : 1);
         if (DT) {
           const uint16_t e16 = (uint16_t)(emax + 1023);
-          a.exps[row] = (uint8_t)(e16 & 0xff);
-          a.exps[row + 1] = (uint8_t)(e16 >> 8);
+          gm(a.exps)[row] = (uint8_t)(e16 & 0xff);
+          gm(a.exps)[row + 1] = (uint8_t)(e16 >> 8);
         } else {
-          a.exps[row] = (uint8_t)(emax + 127);
+          gm(a.exps)[row] = (uint8_t)(emax + 127);
         }
         if (np > 0) {
           if constexpr (kSeg) {
@@ -547,7 +547,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   if (nk > kMaxKeys) {
     if (p.tid == 0) {
       atomicOr(&a.counters[2], 1ull);
-      a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0};
+      gm_store(a.hdrs + (size_t)dense * k.nsb + sb, SbHdr{0, 0, 0});
     }
     __syncthreads();
     return;
@@ -593,7 +593,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
       h.klo = klo;
       h.nkeys = over ? 0 : nk;
       h.ebase = ebs;
-      a.hdrs[(size_t)dense * k.nsb + sb] = h;
+      gm_store(a.hdrs + (size_t)dense * k.nsb + sb, h);
     }
   } else {
     for (int kk = p.w - 1; kk < nk; kk += kTh / 64 - 1) {
@@ -624,7 +624,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     e.size = (C.kbits[ring] + 7) >> 3;
     e.first_block = (uint16_t)(C.kfirst[ring] == 0x7fffffff ? 0xffff : C.kfirst[ring]);
     e.key = (uint16_t)(klo + kk);
-    a.entries[C.ebase + kk] = e;
+    gm_store(a.entries + C.ebase + kk, e);
   }
   // ---- 5. emission ----
   uint32_t* words = staged ? C.v.stream : gwords;
@@ -649,7 +649,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   });
   __syncthreads();
   if (staged)
-    for (uint32_t i = p.tid; i < nwords; i += kTh) gwords[i] = words[i];
+    for (uint32_t i = p.tid; i < nwords; i += kTh) gm(gwords)[i] = words[i];
   __syncthreads();
 }
 

@@ -174,7 +174,7 @@ __device__ __forceinline__ void unstage_subband_even(double (&c)[kN], Lds& S, co
 // at z = 2k + 1); the columns take them after every subband is decoded.
 __device__ __forceinline__ void park_subband_odd(double* Z1, Lds& S, const double* V, const Pos& p, int hx, int hy) {
   const int dx = 17 - hx, dy = 17 - hy;
-  double* dst = Z1 + (hx + 2 * hy) * 16 * 256;
+  const auto dst = gm(Z1) + (hx + 2 * hy) * 16 * 256;
   for (int t = p.tid; t < 16 * 256; t += kTh) {
     const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
     dst[t] = V[(k * dy + j) * dx + i];
@@ -183,7 +183,7 @@ __device__ __forceinline__ void park_subband_odd(double* Z1, Lds& S, const doubl
 }
 
 __device__ __forceinline__ void unpark_odd(double (&c)[kN], const double* Z1, const Pos& p) {
-  const double* q = Z1 + ((p.x & 1) + 2 * (p.y & 1)) * 16 * 256 + (p.y >> 1) * 16 + (p.x >> 1);
+  const auto q = gm(Z1) + ((p.x & 1) + 2 * (p.y & 1)) * 16 * 256 + (p.y >> 1) * 16 + (p.x >> 1);
 #pragma unroll
   for (int k = 0; k < 16; ++k) c[2 * k + 1] = q[k * 256];
 }
@@ -336,7 +336,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
   const int ngr = sb_ok ? (khi - klo) / 32 + 1 : 0;
   const int ng = max(__builtin_amdgcn_readlane(ngr, 0), __builtin_amdgcn_readlane(ngr, 32));
   const uint32_t* blocks = s_blocks + sbi * MB;
-  u64* xs = a.xs + ((size_t)bi * 8 + sbi) * 128 * npl;
+  const auto xs = gm(a.xs) + ((size_t)bi * 8 + sbi) * 128 * npl;
   for (int gi = 0; gi < ng; ++gi) {
     const int g0 = khi - 32 * gi;
     const bool gact = sb_ok && g0 >= klo;
@@ -347,7 +347,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
     if (staged) {
       if (kact && kk < kWalkSlots && W.ssize[sbi][kk]) dentry = 1;
     } else if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
-      dentry = a.dir[((size_t)sbi * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
+      dentry = gm(a.dir)[((size_t)sbi * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
     }
     const bool have = dentry != 0;
     const uint32_t* w32 = staged ? W.sbuf : reinterpret_cast<const uint32_t*>(a.blob);
@@ -401,7 +401,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
     }
   }
   __syncthreads();
-  for (int i = tid; i < NSB * MB; i += TH) a.nbps[(size_t)bi * 1024 + (i / MB) * 128 + i % MB] = (&W.nbps[0][0])[i];
+  for (int i = tid; i < NSB * MB; i += TH) gm(a.nbps)[(size_t)bi * 1024 + (i / MB) * 128 + i % MB] = (&W.nbps[0][0])[i];
 }
 
 // Step 3 for subband sb: plane words -> values in D.v.V, every sample of
@@ -416,7 +416,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   const int nblk = a.blk_n[sb];
   const uint32_t* blocks = s_blocks + sb * 128;
-  const u64* xs = XS + (size_t)sb * 128 * npl;
+  const auto xs = gm(XS) + (size_t)sb * 128 * npl;
   i64* T = &D.T[p.w][0][0];
   for (int g = p.w; g * 4 < nblk; g += kTh / 64) {
     // this lane's row-mode block

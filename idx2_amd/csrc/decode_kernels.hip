@@ -317,7 +317,7 @@ __device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, con
       for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
         const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
         D.v.V[(z * 17 + y) * 17 + x] =
-            a.pgrid[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
+            gm(a.pgrid)[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
       }
     }
     b32::inverse_blocks<DT>(a, sb, dx, dy, D, p, s_blocks, XS);
@@ -404,19 +404,37 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
         const int olast = a.og_f[2] + (a.og_d[2] - 1) * a.og_s[2];
         const size_t row = (size_t)o[1] * a.og_d[0] + o[0];
         const size_t plane = (size_t)a.og_d[1] * a.og_d[0];
+        const int gp0 = b3[2] * a.B3[2], s2 = a.og_s[2], bs = a.bstride;
+        if (bs % s2 == 0) {
+          // the sample stride is a multiple of the output stride: the
+          // divisibility test is the same for every z (no division per z)
+          if ((gp0 - a.og_f[2]) % s2 == 0) {
+            const long long o0 = (gp0 - a.og_f[2]) / s2, st = bs / s2;
 #pragma unroll
-        for (int z = 0; z < 32; ++z) {
-          const int gp = b3[2] * a.B3[2] + z * a.bstride;
-          if (gp >= a.og_f[2] && gp <= olast && (gp - a.og_f[2]) % a.og_s[2] == 0) {
-            const size_t oi = (size_t)((gp - a.og_f[2]) / a.og_s[2]) * plane + row;
-            if (DT) ((double*)a.out)[oi] = c[z];
-            else ((float*)a.out)[oi] = (float)c[z];
+            for (int z = 0; z < 32; ++z) {
+              const int gp = gp0 + z * bs;
+              if (gp >= a.og_f[2] && gp <= olast) {
+                const size_t oi = (size_t)(o0 + z * st) * plane + row;
+                if (DT) gm((double*)a.out)[oi] = c[z];
+                else gm((float*)a.out)[oi] = (float)c[z];
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int z = 0; z < 32; ++z) {
+            const int gp = gp0 + z * bs;
+            if (gp >= a.og_f[2] && gp <= olast && (gp - a.og_f[2]) % s2 == 0) {
+              const size_t oi = (size_t)((gp - a.og_f[2]) / s2) * plane + row;
+              if (DT) gm((double*)a.out)[oi] = c[z];
+              else gm((float*)a.out)[oi] = (float)c[z];
+            }
           }
         }
       }
     } else {
       const size_t plane = (size_t)a.cgdims[1] * a.cgdims[0];
-      double* dst = a.cgrid + ((size_t)(b3[2] * 32) * a.cgdims[1] + (b3[1] * 32 + p.y)) * a.cgdims[0] + (b3[0] * 32 + p.x);
+      const auto dst = gm(a.cgrid) + ((size_t)(b3[2] * 32) * a.cgdims[1] + (b3[1] * 32 + p.y)) * a.cgdims[0] + (b3[0] * 32 + p.x);
 #pragma unroll
       for (int z = 0; z < 32; ++z) dst[z * plane] = c[z];
     }

@@ -350,6 +350,24 @@ __device__ __forceinline__ int wave_min_i(int v) {
   for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
   return v;
 }
+// Global-memory view of a pointer.  Loads and stores through it use the
+// global instructions, which count only against vmcnt; the flat ones used
+// for a pointer of unknown address space also count against lgkmcnt, so a
+// later wait for an LDS result (every barrier has one) would wait for them.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gm(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+// Store of a trivially copyable struct through the global view.
+template <class T>
+__device__ __forceinline__ void gm_store(T* p, const T& v) {
+  static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "32-bit words");
+  auto d = gm(reinterpret_cast<uint32_t*>(p));
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(&v);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 4; ++i) d[i] = s[i];
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
   return v;

@@ -397,11 +397,11 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
     // uniform base of the brick + 32-bit lane/z offsets
     const size_t base = ((size_t)from[2] * a.gdims[1] + from[1]) * (size_t)a.gdims[0] + from[0];
     if (a.in_dtype) {
-      const double* src = (const double*)a.input + base;
+      const auto src = gm((const double*)a.input) + base;
 #pragma unroll
       for (int z = 0; z < 32; ++z) c[z] = src[lo + (unsigned)z * plane];
     } else {
-      const float* src = (const float*)a.input + base;
+      const auto src = gm((const float*)a.input) + base;
 #pragma unroll
       for (int z = 0; z < 32; ++z) c[z] = (double)src[lo + (unsigned)z * plane];
     }
@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
     b32::forward(c, S, p);
     if (!a.coarsest && !(p.x & 1) && !(p.y & 1)) {
       const unsigned pplane = (unsigned)a.pdims[1] * (unsigned)a.pdims[0];
-      double* dst = a.parents + ((size_t)(b3[2] * 16) * a.pdims[1] + b3[1] * 16) * a.pdims[0] + b3[0] * 16;
+      const auto dst = gm(a.parents) + ((size_t)(b3[2] * 16) * a.pdims[1] + b3[1] * 16) * a.pdims[0] + b3[0] * 16;
       const unsigned po = (unsigned)(p.y >> 1) * (unsigned)a.pdims[0] + (unsigned)(p.x >> 1);
 #pragma unroll
       for (int z = 0; z < 16; ++z) dst[po + (unsigned)z * pplane] = c[2 * z];
