@@ -106,7 +106,7 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
       if (nk > kMaxKeys) {
         if (lane == 0) {
           atomicOr(&a.counters[2], 1ull);
-          a.hdrs[(size_t)dense * k.nsb + sb] = SbHdr{0, 0, 0};
+          gm_store(a.hdrs + (size_t)dense * k.nsb + sb, SbHdr{0, 0, 0});
           C.total_s[g] = 0;
           C.woff_s[g] = woff;
           C.base_s[g] = 0;
@@ -154,7 +154,7 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
         h.klo = klo;
         h.nkeys = over ? 0 : nk;
         h.ebase = ebs;
-        a.hdrs[(size_t)dense * k.nsb + sb] = h;
+        gm_store(a.hdrs + (size_t)dense * k.nsb + sb, h);
       }
       woff += (carry + 3) >> 2;
     }
@@ -174,7 +174,7 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
       e.size = (C.kbits[g][kk] + 7) >> 3;
       e.first_block = (uint16_t)(C.kfirst[g][kk] < 0 ? 0xffff : C.kfirst[g][kk]);
       e.key = (uint16_t)(klo + kk);
-      a.entries[C.ebase_s[g] + kk] = e;
+      gm_store(a.entries + C.ebase_s[g] + kk, e);
     }
     if (!staged) {
       uint32_t* gw = (uint32_t*)(a.arena + C.base_s[g]);

@@ -39,9 +39,9 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   const int nblk = G * nb1;
   i64* T = &D.T[p.w][0][0];
-  auto xs_of = [&](int blk) -> const u64* {  // plane words of group block blk
+  auto xs_of = [&](int blk) {  // plane words of group block blk
     const int g = blk / nb1, lb = blk - g * nb1;
-    return a.xs + ((size_t)(bi0 + s0 + g) * 8 + sb) * 128 * npl + (size_t)lb * npl;
+    return gm(a.xs) + ((size_t)(bi0 + s0 + g) * 8 + sb) * 128 * npl + (size_t)lb * npl;
   };
   auto nbps_of = [&](int blk) -> int {
     const int g = blk / nb1, lb = blk - g * nb1;
@@ -143,7 +143,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
 // The group's subband from V to the workgroup's parking area:
 // P[(s * dy + j) * dx + i] for slot s = s0 + g (P: the subband's area).
 __device__ __forceinline__ void park_group(const DecLds2& D, double* P, const Pos2& p, int dx, int dy, int s0, int G) {
-  double* q = P + (size_t)s0 * dx * dy;
+  const auto q = gm(P) + (size_t)s0 * dx * dy;
   for (int t = p.tid; t < G * dx * dy; t += kTh) q[t] = D.V[t];
 }
 
@@ -153,12 +153,12 @@ __device__ __forceinline__ void unpark_odd(double (&c)[kN], double* ex, const do
                                            const Pos2& p) {
   const int hx = p.x & 1;
   const int dx = 17 - hx;
-  const double* q = P + off[hx] + (p.slot * 16) * dx + (p.x >> 1);
+  const auto q = gm(P) + off[hx] + (p.slot * 16) * dx + (p.x >> 1);
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) c[2 * jj + 1] = q[jj * dx];
   for (int t = p.tid; t < kSlots * 16; t += kTh) {
     const int s = t >> 4, jj = t & 15;
-    ex[s * kN + 2 * jj + 1] = P[off[0] + (s * 16 + jj) * 17 + 16];
+    ex[s * kN + 2 * jj + 1] = gm(P)[off[0] + (s * 16 + jj) * 17 + 16];
   }
 }
 

@@ -525,7 +525,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restric
         for (int t = p.tid; t < G * 256; t += b2d::kTh) {
           const int gg = t >> 8, x = t & 15, y = (t >> 4) & 15;
           D.V[(gg * dy + y) * dx + x] =
-              a.pgrid[(size_t)(s_b3[s0 + gg][1] * 16 + y) * a.pgdims[0] + (s_b3[s0 + gg][0] * 16 + x)];
+              gm(a.pgrid)[(size_t)(s_b3[s0 + gg][1] * 16 + y) * a.pgdims[0] + (s_b3[s0 + gg][0] * 16 + x)];
         }
       }
       b2d::inverse_blocks_2d<DT>(a, sb, s0, G, dx, dy, D, pb, s_blocks + sb * 32, a.blk_n[sb], bi0, D.V);
@@ -586,13 +586,13 @@ __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restric
             const int gy = by * a.B3[1] + y * a.bstride;
             if (gy >= a.og_f[1] && gy <= ylast && (gy - a.og_f[1]) % a.og_s[1] == 0) {
               const size_t oi = (size_t)((gy - a.og_f[1]) / a.og_s[1]) * a.og_d[0] + ox;
-              if (DT) ((double*)a.out)[oi] = c[y];
-              else ((float*)a.out)[oi] = (float)c[y];
+              if (DT) gm((double*)a.out)[oi] = c[y];
+              else gm((float*)a.out)[oi] = (float)c[y];
             }
           }
         }
       } else {
-        double* dst = a.cgrid + (size_t)(by * 32) * a.cgdims[0] + (bx * 32 + p.x);
+        const auto dst = gm(a.cgrid) + (size_t)(by * 32) * a.cgdims[0] + (bx * 32 + p.x);
 #pragma unroll
         for (int y = 0; y < 32; ++y) dst[(size_t)y * a.cgdims[0]] = c[y];
       }

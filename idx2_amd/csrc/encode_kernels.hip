@@ -486,11 +486,11 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
       const unsigned row = (unsigned)a.gdims[0];
       const size_t base = (size_t)(by * 32) * a.gdims[0] + (size_t)bx * 32 + p.x;
       if (a.in_dtype) {
-        const double* src = (const double*)a.input + base;
+        const auto src = gm((const double*)a.input) + base;
 #pragma unroll
         for (int y = 0; y < 32; ++y) c[y] = has ? src[(size_t)y * row] : 0.0;
       } else {
-        const float* src = (const float*)a.input + base;
+        const auto src = gm((const float*)a.input) + base;
 #pragma unroll
         for (int y = 0; y < 32; ++y) c[y] = has ? (double)src[(size_t)y * row] : 0.0;
       }
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
     b2d::forward(c, S.ex, p);
     // subband 0 interior into the next level's grid (EncodeBrick, idx2Encode.cpp:303-333)
     if (!a.coarsest && has && !(p.x & 1)) {
-      double* dst = a.parents + (size_t)(by * 16) * a.pdims[0] + (size_t)bx * 16 + (p.x >> 1);
+      const auto dst = gm(a.parents) + (size_t)(by * 16) * a.pdims[0] + (size_t)bx * 16 + (p.x >> 1);
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) dst[(size_t)jj * a.pdims[0]] = c[2 * jj];
     }
