@@ -197,7 +197,7 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
     const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
     const uint32_t lno = b32::lno_at(C, npl, b, pp);
     u64 lo, hi;
-    b32::plane_bits(*b32::plane_at(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
+    b32::plane_bits(b32::plane_load(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
     const int kk = (63 - pp) + e + kBpBiasDev - C.klo_s[g];
     const u64 pos = (u64)C.kbyte[g][kk] * 8 + (lno >> 15);
     if (staged) b32::or_bits128<true>(C.v.stream + C.woff_s[g], pos, lo, hi);

@@ -342,9 +342,16 @@ __device__ __forceinline__ void stage_subband_odd(const double* Z1, const Lds& S
 
 template <class CL>
 __device__ __forceinline__ uint32_t& lno_at(CL& C, int npl, int b, int pp) { return C.pool[b * npl + pp]; }
+// Plane word of (block b, plane pp): in LDS after the pool entries for f32
+// (npl 32), in the workgroup's global scratch XG for f64.
 template <class CL>
-__device__ __forceinline__ u64* plane_at(CL& C, u64* XG, int npl, int b, int pp) {
-  return npl == 32 ? reinterpret_cast<u64*>(C.pool + kMaxBlocks * 32) + b * 32 + pp : XG + b * 64 + pp;
+__device__ __forceinline__ void plane_store(CL& C, u64* XG, int npl, int b, int pp, u64 X) {
+  if (npl == 32) reinterpret_cast<u64*>(C.pool + kMaxBlocks * 32)[b * 32 + pp] = X;
+  else gm(XG)[b * 64 + pp] = X;
+}
+template <class CL>
+__device__ __forceinline__ u64 plane_load(const CL& C, const u64* XG, int npl, int b, int pp) {
+  return npl == 32 ? reinterpret_cast<const u64*>(C.pool + kMaxBlocks * 32)[b * 32 + pp] : gm(XG)[b * 64 + pp];
 }
 
 // Coefficient c of block slot qq (coefficient order, then negabinary),
@@ -373,7 +380,7 @@ __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bo
     const int n_in = n;
     const int len = plane_len_cf(X, n, 1 << (2 * nd));
     lno_at(C, npl, blk, pp) = (uint32_t)len | ((uint32_t)n_in << 8);
-    *plane_at(C, XG, npl, blk, pp) = X;
+    plane_store(C, XG, npl, blk, pp, X);
     if constexpr (kAcc) {
       const int ring = ((63 - pp) + e + kBpBiasDev) & (kMaxKeys - 1);
       atomicAdd(&C.kbits[ring], (uint32_t)len);
@@ -641,7 +648,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
     const uint32_t lno = lno_at(C, npl, b, pp);
     u64 lo, hi;
-    plane_bits(*plane_at(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
+    plane_bits(plane_load(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
     const int ring = ((63 - pp) + e + kBpBiasDev) & M;
     const u64 pos = (u64)C.kbyte[ring] * 8 + (lno >> 15);
     if (staged) or_bits128<true>(words, pos, lo, hi);
