@@ -140,7 +140,7 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
         cur.aleft -= alloc;
         cur.ebase += nk;
         cur.eleft -= nk;
-        if (carry) atomicAdd(&a.counters[3], (unsigned long long)carry);
+        cur.bytes += carry;
         const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
         if (over) {
           atomicOr(&a.counters[2], 2ull);

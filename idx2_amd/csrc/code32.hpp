@@ -33,7 +33,15 @@ constexpr unsigned long long kEntryChunk = 2048;          // stream entries a wo
 // with one global atomic when exhausted (no atomic round trip per subband).
 struct ArenaCursor {
   unsigned long long abase, aleft, ebase, eleft;
+  unsigned long long bytes;  // stream bytes coded, added to counters[3] once at the end
 };
+
+// The workgroup's stream byte count into counters[3] (all jobs of a launch
+// share the counters), once per kernel instead of one atomic per subband.
+__device__ __forceinline__ void flush_cursor(const ArenaCursor& cur, unsigned long long* counters) {
+  __syncthreads();
+  if (threadIdx.x == 0 && cur.bytes) atomicAdd(&counters[3], cur.bytes);
+}
 
 struct CodeLds {
   union {
@@ -589,7 +597,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
       cur.aleft -= alloc;
       cur.ebase += nk;
       cur.eleft -= nk;
-      if (carry) atomicAdd(&a.counters[3], (unsigned long long)carry);
+      cur.bytes += carry;
       const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
       if (over) atomicOr(&a.counters[2], 2ull);
       C.base = bse;

@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
   __shared__ uint32_t s_blocks[8 * 128];
   __shared__ b32::ArenaCursor cur;
   b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
-  if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0};
+  if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0, 0};
   const int total = start[njobs];
   int cj = -1;
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
@@ -433,6 +433,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
                     reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride), s_blocks, cur);
     __syncthreads();
   }
+  b32::flush_cursor(cur, table[0].counters);
 }
 
 // Full 2D 32x32 bricks, 32 per workgroup iteration (one per 32-lane
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
   __shared__ b32::ArenaCursor cur;
   b2d::Lds2& S = *reinterpret_cast<b2d::Lds2*>(lds_raw);
   b2d::CodeLds2& C = S.u.code;
-  if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0};
+  if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0, 0};
   const int total = start[njobs];
   int cj = -1;
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
@@ -583,6 +584,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
     }
     __syncthreads();
   }
+  b32::flush_cursor(cur, table[0].counters);
 }
 
 // --------------------------------------------------------------------------
