@@ -32,15 +32,6 @@ struct DecLds32 {
   unsigned char p3[64], p2[16];
 };
 
-// 64 bits of a byte stream at bit pos (words padded by >= 8 bytes)
-__device__ __forceinline__ u64 peek64w(const uint32_t* w32, u64 pos) {
-  const uint32_t* w = w32 + (pos >> 5);
-  const int sh = int(pos & 31);
-  const u64 lo = (u64)w[0] | ((u64)w[1] << 32);
-  if (!sh) return lo;
-  return (lo >> sh) | ((u64)w[2] << (64 - sh));
-}
-
 // 128 bits of one lane's stream kept in registers: every lane of a walk
 // reads a different stream, so each load instruction touches 64 cache lines;
 // a 128-bit refill serves two to three planes instead of two 64-bit windows
@@ -100,28 +91,6 @@ __device__ __forceinline__ u64 decode_plane_win(const uint32_t* w32, BitBuf& B, 
     pos += adv;
     w = adv >= 64 ? 0 : w >> adv;
     avail -= adv;
-    X |= 1ull << n;
-    ++n;
-  }
-  return X;
-}
-
-__device__ __forceinline__ u64 decode_plane_w(const uint32_t* w32, u64& pos, int& n, int nv) {
-  u64 X = 0;
-  if (n > 0) {
-    const u64 w = peek64w(w32, pos);
-    X = n >= 64 ? w : (w & ((1ull << n) - 1));
-    pos += n;
-  }
-  while (n < nv) {
-    u64 w = peek64w(w32, pos);
-    if (!(w & 1)) { pos += 1; break; }  // group insignificant
-    pos += 1;
-    w >>= 1;
-    const int lim = nv - 1 - n;  // bits examined before the implicit last one
-    const int z = w ? __ffsll((long long)w) - 1 : 64;
-    if (z < lim) { n += z; pos += z + 1; }
-    else { pos += lim; n = nv - 1; }
     X |= 1ull << n;
     ++n;
   }
