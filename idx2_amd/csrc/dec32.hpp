@@ -32,6 +32,33 @@ struct DecLds32 {
   unsigned char p3[64], p2[16];
 };
 
+// Past the decode kernel's LDS (faces + DecLds32): the brick's parent
+// subband-0 interior, 16^3 doubles [z][y][x], loaded straight into LDS when
+// the brick starts and copied into V when subband 0 is decoded.
+constexpr size_t dec_parent_off() {
+  return (offsetof(Lds, u) +
+          (sizeof(DecLds32) > sizeof(Lds) - offsetof(Lds, u) ? sizeof(DecLds32) : sizeof(Lds) - offsetof(Lds, u)) + 15) &
+         ~size_t(15);
+}
+constexpr size_t kDecParentBytes = 16 * 16 * 16 * sizeof(double);
+
+// Issue the loads of the parent interior of brick b3 into PB: rows
+// r = z * 16 + y of 16 doubles, eight rows (1 KB) per wave instruction, each
+// lane 16 bytes (global_load_lds_dwordx4: no registers held while the loads
+// are in flight).  The caller waits (vmcnt) before reading PB.
+__device__ __forceinline__ void load_parent_lds(const double* pgrid, const int* pgdims, const int (&b3)[3], double* PB,
+                                                const Pos& p) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int r0 = p.w * 16 + h * 8;
+    const int r = r0 + (p.lane >> 3), z = r >> 4, y = r & 15;
+    const double* src = pgrid + ((size_t)(b3[2] * 16 + z) * pgdims[1] + (b3[1] * 16 + y)) * pgdims[0] + b3[0] * 16 +
+                        (p.lane & 7) * 2;
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)(PB + r0 * 16), 16, 0, 0);
+  }
+}
+
 // 128 bits of one lane's stream kept in registers: every lane of a walk
 // reads a different stream, so each load instruction touches 64 cache lines;
 // a 128-bit refill serves two to three planes instead of two 64-bit windows

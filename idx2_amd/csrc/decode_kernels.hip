@@ -303,11 +303,13 @@ __global__ void __launch_bounds__(WS::th) k_walk(const DecArgs* __restrict__ tab
 // stage of the walked planes.
 template <int DT>
 __device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, const int (&b3)[3], b32::DecLds32& D,
-                                                   const b32::Pos& p, const uint32_t* s_blocks, const u64* XS) {
+                                                   const b32::Pos& p, const uint32_t* s_blocks, const u64* XS,
+                                                   const double* PB = nullptr) {
   const BrickConst& k = a.k;
   const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
   const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
-  __syncthreads();  // V free
+  if (PB) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's parent rows are in LDS
+  __syncthreads();  // V free (and every wave's parent rows in PB)
   if (!((a.mask >> sb) & 1)) {  // subband not decoded: zeros
     for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
   } else {
@@ -317,7 +319,8 @@ __device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, con
       for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
         const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
         D.v.V[(z * 17 + y) * 17 + x] =
-            gm(a.pgrid)[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
+            PB ? PB[i]
+               : gm(a.pgrid)[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
       }
     }
     b32::inverse_blocks<DT>(a, sb, dx, dy, D, p, s_blocks, XS);
@@ -364,6 +367,13 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
     const int dense = a.dense[bi];
     const u64* XS = a.xs + (size_t)bi * 8 * 128 * npl;
+    // the parent interior of subband 0 on its way into LDS while the z-odd
+    // subbands decode (16-byte aligned rows only; otherwise read when used)
+    double* PB = nullptr;
+    if (!a.coarsest && (a.mask & 1) && ((uintptr_t)a.pgrid & 15) == 0 && (a.pgdims[0] & 1) == 0) {
+      PB = reinterpret_cast<double*>(lds_raw + b32::dec_parent_off());
+      b32::load_parent_lds(a.pgrid, a.pgdims, b3, PB, p);
+    }
     if (p.tid < 64) D.p3[p.tid] = a.perm.p3[p.tid];
     if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
     // exponents and decoded plane counts of the brick's blocks
@@ -382,7 +392,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
     double c[b32::kN];
     for (int sb = 0; sb < 8; ++sb) {
       if (k.sb[sb].f[2] != 0) continue;
-      decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS);
+      decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS, sb == 0 ? PB : nullptr);
       b32::unstage_subband_even(c, S, D.v.V, p, k.sb[sb].f[0], k.sb[sb].f[1]);
       __syncthreads();
     }
