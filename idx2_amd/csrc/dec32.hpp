@@ -198,16 +198,21 @@ constexpr int kWalkSlots = 64;                // staged streams per subband
 // 24.4 ms, 20 KB 25.5 ms, 44 KB 29.3 ms).
 constexpr int kWalkStreamBytes = 4 * 1024;
 
-template <int NSB, int MB, int TH, int SBYTES>
+template <int NSB, int MB, int TH, int SBYTES, int OCC>
 struct WalkShape {
   static constexpr int nsb = NSB;        // subbands
   static constexpr int mb = MB;          // blocks per subband (power of two)
   static constexpr int th = TH;          // threads: two subbands per wave
   static constexpr int sbytes = SBYTES;  // LDS stream buffer
+  // waves per SIMD the register allocation is held to (the walk is VALU
+  // bound and latency hidden by occupancy; measured on the 1024^3 float32
+  // decode: 5 waves (92 VGPRs) 22.15 ms, 6 21.59 ms, 7 (72 VGPRs, the LDS
+  // limit of the 3D shape) 21.31 ms; the 2D shape spills less at 6)
+  static constexpr int occ = OCC;
   static_assert(2 * (TH / 64) == NSB, "one wave per two subbands");
 };
-using Walk3D = WalkShape<8, 128, kWalkThreads, kWalkStreamBytes>;
-using Walk2D = WalkShape<4, 32, 128, 2048>;
+using Walk3D = WalkShape<8, 128, kWalkThreads, kWalkStreamBytes, 7>;
+using Walk2D = WalkShape<4, 32, 128, 2048, 6>;
 
 template <class WS>
 struct WalkLds {

@@ -276,7 +276,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
 // WS (b32::Walk3D for 32^3 bricks, b32::Walk2D for 32x32 bricks); plane words
 // and counts to the batch arrays a.xs / a.nbps.
 template <class WS>
-__global__ void __launch_bounds__(WS::th) k_walk(const DecArgs* __restrict__ table, const int* __restrict__ start,
+__global__ void __launch_bounds__(WS::th, WS::occ) k_walk(const DecArgs* __restrict__ table, const int* __restrict__ start,
                                                  int njobs) {
   __shared__ b32::WalkLds<WS> W;
   __shared__ uint32_t s_blocks[WS::nsb * WS::mb];
