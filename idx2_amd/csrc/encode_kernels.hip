@@ -791,9 +791,25 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
     i = 0;
     do { uint8_t byte = v & 0x7f; v >>= 7; out[vn + i++] = byte | (v ? 0x80 : 0); } while (v);
   }
+  // this thread's brick of the step: its stream (size 0: absent)
+  const u64 b = s0 + tid;
+  uint32_t sz = 0;
+  u64 src = 0;
+  if (b < s1e) {
+    const uint32_t e = row[b];
+    if (e) { sz = entries[e - 1].size; src = entries[e - 1].off; }
+  }
   // unary gaps as a presence bitmap: byte j holds the bits of bricks
   // first_brick + 1 + 8 (j - vf) + i and belongs to the step that holds the
-  // brick of its first bit
+  // brick of its first bit; presence of the step's bricks (and of the seven
+  // after it, for the last byte) comes from one lookup per brick, via LDS
+  __shared__ uint8_t s_pres[kChunkStep + 8];
+  s_pres[tid] = sz > 0;
+  if (tid < 8) {
+    const u64 bx = s1e + tid;
+    s_pres[kChunkStep + tid] = bx < b1 && present(bx);
+  }
+  __syncthreads();
   {
     const long long f1 = (long long)c.first_brick + 1;
     const long long lo = ((long long)s0 - f1 + 7) >> 3, hi = ((long long)s1e - f1 + 7) >> 3;
@@ -803,7 +819,10 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
       uint8_t byte = 0;
       for (int i = 0; i < 8; ++i) {
         const u64 bit = 8 * (j - vf) + i;
-        if (bit < nbm && present(c.first_brick + 1 + bit)) byte |= uint8_t(1u << i);
+        // brick f1 + bit sits at s_pres[f1 + bit - s0] (in [0, kChunkStep + 8))
+        const long long at = f1 + (long long)bit - (long long)s0;
+        if (bit < nbm && at >= 0 && at < s1e - s0 + 8 && s_pres[at < (long long)(s1e - s0) ? at : kChunkStep + (at - (long long)(s1e - s0))])
+          byte |= uint8_t(1u << i);
       }
       out[vn + j] = byte;
     }
@@ -816,13 +835,6 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
   const u64 sizes_off = (u64)vn + dbytes;
   const u64 payload_start = sizes_off + so[nsteps].x;
   {
-    const u64 b = s0 + tid;
-    uint32_t sz = 0;
-    u64 src = 0;
-    if (b < s1e) {
-      const uint32_t e = row[b];
-      if (e) { sz = entries[e - 1].size; src = entries[e - 1].off; }
-    }
     const uint32_t v1 = sz ? (uint32_t)varbyte_len(sz) : 0u;
     // exclusive prefix sums over the step's bricks: wave scans + wave totals
     const uint32_t i1 = b32::wave_scan_add(v1, lane), i2 = b32::wave_scan_add(sz, lane);
