@@ -89,39 +89,20 @@ def cpu_baseline(ctx, torch, cfg_name, seed):
                       f"{t1 - t0:.2f}s + serial decode {t2 - t1:.2f}s (includes file I/O)"}
 
 
-# Serving loop workload: resident datasets, Poisson arrivals at a fixed rate,
-# random ROI (edge 32-1024, clipped), downsampling U{0,1,2} per axis,
-# tolerance log-uniform in [1e-6, 1e-2], HBM admission budget 200 GB.
-SERVE_SETS = [((512, 512, 512), "float64"), ((1024, 1024, 1024), "float32"), ((2048, 2048, 512), "float32")]
-
-
 def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=48 << 30):
     import threading
-    from idx2_amd import api
+    from idx2_amd import api, workloads
     sets = []
-    for i, (dims, dt) in enumerate(SERVE_SETS):
+    for i, (dims, dt) in enumerate(workloads.SERVE_SETS):
         n = dims[0] * dims[1] * dims[2]
         buf = torch.empty(n, dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
         ctx.synth_field(buf.data_ptr(), dims, dt, seed + 100 + i)
         ds = api.Dataset.encode(ctx, dev_ptr=buf.data_ptr(), dims=dims, dtype=dt, name="s%d" % i, field="f",
-                                brick=(32, 32, 32), nlevels=3, tolerance=1e-6)
+                                **workloads.SERVE_ENCODE)
         del buf
         sets.append((ds, dims, dt))
     torch.cuda.synchronize()
-    rng = np.random.default_rng(seed + 7)
-    reqs = []
-    for _ in range(n_requests):
-        k = int(rng.integers(0, len(sets)))
-        ds, dims, dt = sets[k]
-        first, last = [], []
-        for d in range(3):
-            e = min(int(rng.integers(32, 1025)), dims[d])
-            f = int(rng.integers(0, dims[d] - e + 1))
-            first.append(f)
-            last.append(f + e - 1)
-        q = dict(first=tuple(first), last=tuple(last), downsampling=tuple(int(x) for x in rng.integers(0, 3, 3)),
-                 tolerance=float(10 ** rng.uniform(-6, -2)))
-        reqs.append((ds, dt, q, float(rng.exponential(1.0 / rate))))
+    reqs = [(sets[k][0], sets[k][2], q, gap) for k, q, gap in workloads.serve_requests(seed, n_requests, rate)]
     server = api.Server(0, hbm_budget=budget, slots=slots)
     lat, qms, out_bytes, read_bytes = [], [], 0, 0
     pending = []
@@ -183,7 +164,7 @@ def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, m
         ds.free()
     lat = np.array(lat)
     return {"requests": n_requests, "rate_per_s": rate, "slots": slots, "hbm_budget_bytes": budget,
-            "datasets": [list(d) + [t] for d, t in SERVE_SETS],
+            "datasets": [list(d) + [t] for d, t in workloads.SERVE_SETS],
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_queue_ms": round(float(np.mean(qms)), 3), "served_per_s": round(n_requests / wall, 2),
             "output_gbps": round(out_bytes / wall / 1e9, 3), "stream_bytes_read": int(read_bytes),
@@ -191,30 +172,18 @@ def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, m
             "client_stalls": stalls}
 
 
-BATCH_DIMS = (64, 96, 128, 192, 256, 384, 512)
-
-
 def batched_fields(torch, ctx, seed, n_fields=64):
-    """The 64-field mix: dims per axis from BATCH_DIMS, 2D and 3D 50/50, float32
-    and float64 50/50, 2-4 levels clamped to each field's maximum, tolerance
-    1e-4, field f generated with seed + f."""
-    from idx2_amd import api
-    rng = np.random.default_rng(seed + 3000)
+    """The 64-field mix (idx2_amd.workloads.batched_specs) generated on the device."""
+    from idx2_amd import workloads
     fields, bufs = [], []
-    for f in range(n_fields):
-        is2d = bool(rng.random() < 0.5)
-        dims = [int(rng.choice(BATCH_DIMS)) for _ in range(3)]
-        if is2d:
-            dims[2] = 1
-        dt = "float64" if rng.random() < 0.5 else "float32"
-        brick = (32, 32, 1) if is2d else (32, 32, 32)
-        nl = max(1, min(int(rng.integers(2, 5)), api.max_levels(dims, brick)))
+    for sp in workloads.batched_specs(seed, n_fields):
+        dims, dt = sp["dims"], sp["dtype"]
         buf = torch.empty(dims[0] * dims[1] * dims[2], dtype=torch.float64 if dt == "float64" else torch.float32,
                           device="cuda")
-        ctx.synth_field(buf.data_ptr(), dims, dt, seed + f)
+        ctx.synth_field(buf.data_ptr(), dims, dt, sp["seed"])
         bufs.append(buf)
-        fields.append(dict(dev_ptr=buf.data_ptr(), dims=tuple(dims), dtype=dt, brick=brick, nlevels=nl,
-                           tolerance=1e-4, name=f"F{f}", field="f"))
+        fields.append(dict(dev_ptr=buf.data_ptr(), dims=dims, dtype=dt, brick=sp["brick"], nlevels=sp["nlevels"],
+                           tolerance=sp["tolerance"], name=sp["name"], field=sp["field"]))
     return fields, bufs
 
 
@@ -223,10 +192,11 @@ def batched_bench(ctx, torch, seed, reps=3):
     path) against one idx2g_encode_batch + one idx2g_decode_batch."""
     from idx2_amd import api
     fields, bufs = batched_fields(torch, ctx, seed)
-    outs = [torch.empty_like(b) for b in bufs]
+    outs_by_mode = {m: [torch.empty_like(b) for b in bufs] for m in ("per_field", "batched")}
     raw = sum(b.numel() * b.element_size() for b in bufs)
     res = {}
     for mode in ("per_field", "batched"):
+        outs = outs_by_mode[mode]
         enc, dec = [], []
         for rep in range(reps + 1):
             torch.cuda.synchronize()
@@ -255,24 +225,23 @@ def batched_bench(ctx, torch, seed, reps=3):
         res[mode] = {"encode_ms": round(e * 1e3, 3), "decode_ms": round(d * 1e3, 3),
                      "gbps": round(raw / (e + d) / 1e9, 3)}
     res["speedup"] = round(res["batched"]["gbps"] / res["per_field"]["gbps"], 3)
-    ok = all(bool(torch.equal(o, b)) or float((o.double() - b.double()).abs().max()) < 1.0 for o, b in zip(outs, bufs))
+    # every batched decode must be bit-identical to the same field's per-field decode
+    mismatched = sum(0 if torch.equal(a.view(torch.uint8), b.view(torch.uint8)) else 1
+                     for a, b in zip(outs_by_mode["per_field"], outs_by_mode["batched"]))
     res.update({"fields": len(fields), "raw_bytes": int(raw), "chunk_bytes": int(chunk),
                 "fields_2d": sum(1 for f in fields if f["dims"][2] == 1),
                 "fields_f64": sum(1 for f in fields if f["dtype"] == "float64"),
-                "levels": [f["nlevels"] for f in fields], "decoded": ok})
-    del bufs, outs
+                "levels": [f["nlevels"] for f in fields], "batch_vs_per_field_mismatches": mismatched})
+    del bufs, outs_by_mode
     return res
-
-
-PROGRESSIVE = dict(dims=(2048, 2048, 1024), dtype="float32", nlevels=3, tolerance=1e-6, brick=(32, 32, 32))
 
 
 def progressive_bench(ctx, torch, seed, n_roi=32):
     """Encode 2048x2048x1024 float32 once; decode at downsampling (2,2,2),
     (1,1,1), (0,0,0) x tolerance {1e-2, 1e-3, 1e-4, 1e-6}, full volume and
     n_roi seeded random boxes (edge 64-512)."""
-    from idx2_amd import api
-    c = PROGRESSIVE
+    from idx2_amd import api, workloads
+    c = workloads.PROGRESSIVE
     dims = c["dims"]
     buf = torch.empty(dims[0] * dims[1] * dims[2], dtype=torch.float32, device="cuda")
     ctx.synth_field(buf.data_ptr(), dims, "float32", seed)
@@ -280,16 +249,7 @@ def progressive_bench(ctx, torch, seed, n_roi=32):
                             brick=c["brick"], nlevels=c["nlevels"], tolerance=c["tolerance"])
     enc_ms = ctx.stats()["total_ms"]
     del buf
-    rng = np.random.default_rng(seed + 11)
-    boxes = [None]
-    for _ in range(n_roi):
-        first, last = [], []
-        for d in range(3):
-            e = int(rng.integers(64, 513))
-            f = int(rng.integers(0, dims[d] - e + 1))
-            first.append(f)
-            last.append(f + e - 1)
-        boxes.append((tuple(first), tuple(last)))
+    boxes = [None] + workloads.progressive_boxes(seed, dims, n_roi)
     rows = []
     out = torch.empty(dims[0] * dims[1] * dims[2], dtype=torch.float32, device="cuda")
     for dsmp in ((2, 2, 2), (1, 1, 1), (0, 0, 0)):

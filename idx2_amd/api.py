@@ -13,15 +13,21 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_build", "libidx2g.so")
 
+# idx2g error codes: the values of the reference's enum class idx2_err_code
+# (Source/Core/idx2Common.h:36-56 after Source/Core/ErrorCodes.h:6-12;
+# idx2_Enum of Source/Core/Enum.h:65-80 puts __Invalid__ last, NoError = 0).
 ERRORS = {
-    0: "NoError", 1: "Unknown", 2: "SizeZero", 3: "SizeTooSmall", 4: "DimensionMismatched",
-    5: "SizeMismatched", 6: "AlignmentError", 7: "OutOfMemory", 8: "TypeNotSupported",
-    11: "FileReadFailed", 12: "FileWriteFailed", 13: "FileCreateFailed", 17: "ParseFailed",
+    0: "NoError", 1: "UnknownError", 2: "SizeZero", 3: "SizeTooSmall", 4: "SizeMismatched",
+    5: "DimensionMismatched", 6: "DimensionsTooMany", 7: "AttributeNotFound", 8: "OptionNotSupported",
+    9: "TypeNotSupported", 10: "FileCreateFailed", 11: "FileReadFailed", 12: "FileWriteFailed",
+    13: "FileOpenFailed", 14: "FileCloseFailed", 15: "FileSeekFailed", 16: "FileTellFailed",
+    17: "ParseFailed", 18: "OutOfMemory", 19: "DimensionsRepeated", 20: "DimensionTooSmall",
     21: "BrickSizeNotPowerOfTwo", 22: "BrickSizeTooBig", 23: "TooManyLevels",
+    24: "TooManyTransformPassesPerLevel", 25: "TooManyLevelsOrTransformPasses", 26: "TooManyBricksPerFile",
     27: "TooManyFilesPerDir", 28: "NotSupportedInVersion", 29: "CannotCreateDirectory",
     30: "SyntaxError", 31: "TooManyBricksPerChunk", 32: "TooManyChunksPerFile",
     33: "ChunksPerFileNotPowerOf2", 34: "BricksPerChunkNotPowerOf2", 35: "ChunkNotFound",
-    36: "BrickNotFound", 37: "FileNotFound", 100: "DeviceError",
+    36: "BrickNotFound", 37: "FileNotFound", 38: "UnsupportedScheme", 100: "DeviceError",
 }
 
 
@@ -357,21 +363,29 @@ class Server:
 
     def __init__(self, device=0, hbm_budget=200 << 30, slots=4):
         self._h = ctypes.c_void_p()
+        self._held = {}  # request id -> (dataset, output owner) kept alive until wait()
         _check(lib().idx2g_server_create(device, int(hbm_budget), int(slots), ctypes.byref(self._h)))
 
-    def submit(self, ds, out_dev_ptr, **query):
-        """Queue a decode of `query` on `ds` into a device buffer; returns the request id."""
+    def submit(self, ds, out_dev_ptr, out_owner=None, **query):
+        """Queue a decode of `query` on `ds` into a device buffer; returns the
+        request id.  `ds` and `out_owner` (e.g. the tensor behind out_dev_ptr)
+        stay referenced until wait() returns for the request."""
         q = make_query(**query)
         rid = ctypes.c_int64()
         _check(lib().idx2g_server_submit(self._h, ds._h, ctypes.byref(q), ctypes.c_void_p(out_dev_ptr), 1,
                                          ctypes.byref(rid)))
+        self._held[rid.value] = (ds, out_owner)
         return rid.value
 
     def wait(self, rid):
         """Block until request `rid` is done: (latency_ms, queue_ms, stats)."""
         lat, qms = ctypes.c_double(), ctypes.c_double()
         st = Stats()
-        _check(lib().idx2g_server_wait(self._h, int(rid), ctypes.byref(lat), ctypes.byref(qms), ctypes.byref(st)))
+        try:
+            _check(lib().idx2g_server_wait(self._h, int(rid), ctypes.byref(lat), ctypes.byref(qms),
+                                           ctypes.byref(st)))
+        finally:
+            self._held.pop(int(rid), None)
         return lat.value, qms.value, {k: getattr(st, k) for k, _ in Stats._fields_}
 
     def info(self):
@@ -381,8 +395,9 @@ class Server:
 
     def close(self):
         if self._h:
-            lib().idx2g_server_destroy(self._h)
+            lib().idx2g_server_destroy(self._h)  # drains the queue first
             self._h = ctypes.c_void_p()
+        self._held.clear()
 
     def __del__(self):
         try:

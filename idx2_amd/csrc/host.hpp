@@ -1,6 +1,8 @@
 // Host-side objects of the idx2g library: context, resident dataset.
 #pragma once
 
+#include <atomic>
+
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -53,9 +55,10 @@ struct idx2g_dataset {
   double* d_minmax = nullptr;    // per level-0 brick min/max (encode only)
   bool have_range = false;
   uint64_t chunk_bytes = 0;
-  // decode scratch kept across queries
-  double* d_scratch = nullptr;
-  size_t scratch_bytes = 0;
+  // serving requests queued or running on this dataset; idx2g_dataset_free
+  // while some are in flight only marks it, and the last one frees it
+  std::atomic<int> inflight{0};
+  std::atomic<bool> free_pending{false}, freed{false};
 };
 
 namespace idx2g {

@@ -101,8 +101,6 @@ int Layout::finalize() {
     if (!is_pow2(brick[d])) return IDX2G_BRICK_SIZE_NOT_POW2;
   for (int d = 0; d < 3; ++d)
     if (dims[d] < brick[d]) return IDX2G_BRICK_SIZE_TOO_BIG;
-  for (int d = 0; d < 3; ++d)  // partial 4^d blocks must have extent 1 (brick >= 8)
-    if (brick[d] > 1 && brick[d] < 8) return IDX2G_NOT_SUPPORTED;
   if (nlevels == 0) {  // GuessNumLevelsIfNeeded, idx2Common.cpp:666-680
     int b[3] = {brick[0], brick[1], brick[2]};
     auto le = [&](const int* v) { return v[0] <= dims[0] && v[1] <= dims[1] && v[2] <= dims[2]; };
@@ -112,7 +110,6 @@ int Layout::finalize() {
     }
   }
   if (nlevels > kMaxLevels || nlevels < 1) return IDX2G_TOO_MANY_LEVELS;
-  if (bppc != 1) return IDX2G_NOT_SUPPORTED;
   // transform order: transformed axes then "++" (idx2Common.cpp:373-387)
   std::string tord;
   for (int d = 0; d < 3; ++d)
@@ -223,6 +220,12 @@ int Layout::finalize() {
     if (fdd[l].size() > 4) return IDX2G_TOO_MANY_FILES_PER_DIR;
     std::reverse(fdd[l].begin(), fdd[l].end());
   }
+  // limits of this implementation, checked after every check of the
+  // reference so that parameters the reference rejects get its code:
+  // partial 4^d blocks must have extent 1 (brick >= 8), one bit plane per chunk
+  for (int d = 0; d < 3; ++d)
+    if (brick[d] > 1 && brick[d] < 8) return IDX2G_NOT_SUPPORTED;
+  if (bppc != 1) return IDX2G_NOT_SUPPORTED;
   // blocks of every subband, Morton order (idx2Encode.cpp:333-350)
   for (int l = 0; l < nlevels; ++l)
     for (int s = 0; s < nsb; ++s) {

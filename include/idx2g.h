@@ -1,9 +1,17 @@
 /* idx2g: MI355X (gfx950) implementation of the idx2 encode/decode path.
  *
  * C ABI: plain pointers and sizes, no C++ or torch types.  Every call returns
- * an idx2 error code (numbering of the reference enum idx2_err_code,
- * Source/Core/idx2Common.h:36-56 after the common codes of
- * Source/Core/ErrorCodes.h:6-12); 0 is success.  No call throws.
+ * an idx2 error code; 0 is success.  No call throws.
+ *
+ * Error numbering: the values of the reference enum idx2_err_code
+ * (Source/Core/idx2Common.h:36-56), whose first 21 members are the common
+ * codes of Source/Core/ErrorCodes.h:6-12.  The reference's effective
+ * idx2_Enum (Source/Core/Enum.h:65-80, which replaces the definition at
+ * Enum.h:14-26) declares an enum class with __Invalid__ last, so NoError is 0
+ * and every code here equals the reference's enum value (checked at compile
+ * time by include/idx2g_idx2.hpp against the reference's headers).
+ * IDX2G_DEVICE_ERROR (100) has no reference counterpart (a HIP runtime
+ * failure; the reference runs on the CPU).
  *
  * A context owns one HIP stream and the device pools on one GPU.  Contexts are
  * not thread safe; use one per host thread.  Datasets own their device memory.
@@ -18,24 +26,35 @@ extern "C" {
 #endif
 
 enum idx2g_err {
-  IDX2G_OK = 0,
-  IDX2G_UNKNOWN = 1,
+  IDX2G_OK = 0,                       /* NoError */
+  IDX2G_UNKNOWN = 1,                  /* UnknownError */
   IDX2G_SIZE_ZERO = 2,
   IDX2G_SIZE_TOO_SMALL = 3,
-  IDX2G_DIMENSION_MISMATCHED = 4,
-  IDX2G_SIZE_MISMATCHED = 5,
-  IDX2G_ALIGNMENT_ERROR = 6,
-  IDX2G_OUT_OF_MEMORY = 7,
-  IDX2G_TYPE_NOT_SUPPORTED = 8,
+  IDX2G_SIZE_MISMATCHED = 4,
+  IDX2G_DIMENSION_MISMATCHED = 5,
+  IDX2G_DIMENSIONS_TOO_MANY = 6,
+  IDX2G_ATTRIBUTE_NOT_FOUND = 7,
+  IDX2G_OPTION_NOT_SUPPORTED = 8,
+  IDX2G_TYPE_NOT_SUPPORTED = 9,
+  IDX2G_FILE_CREATE_FAILED = 10,
   IDX2G_FILE_READ_FAILED = 11,
   IDX2G_FILE_WRITE_FAILED = 12,
-  IDX2G_FILE_CREATE_FAILED = 13,
+  IDX2G_FILE_OPEN_FAILED = 13,
+  IDX2G_FILE_CLOSE_FAILED = 14,
+  IDX2G_FILE_SEEK_FAILED = 15,
+  IDX2G_FILE_TELL_FAILED = 16,
   IDX2G_PARSE_FAILED = 17,
+  IDX2G_OUT_OF_MEMORY = 18,
+  IDX2G_DIMENSIONS_REPEATED = 19,
+  IDX2G_DIMENSION_TOO_SMALL = 20,
   IDX2G_BRICK_SIZE_NOT_POW2 = 21,
   IDX2G_BRICK_SIZE_TOO_BIG = 22,
   IDX2G_TOO_MANY_LEVELS = 23,
+  IDX2G_TOO_MANY_TRANSFORM_PASSES_PER_LEVEL = 24,
+  IDX2G_TOO_MANY_LEVELS_OR_TRANSFORM_PASSES = 25,
+  IDX2G_TOO_MANY_BRICKS_PER_FILE = 26,
   IDX2G_TOO_MANY_FILES_PER_DIR = 27,
-  IDX2G_NOT_SUPPORTED = 28,
+  IDX2G_NOT_SUPPORTED = 28,           /* NotSupportedInVersion */
   IDX2G_CANNOT_CREATE_DIRECTORY = 29,
   IDX2G_SYNTAX_ERROR = 30,
   IDX2G_TOO_MANY_BRICKS_PER_CHUNK = 31,
@@ -45,6 +64,7 @@ enum idx2g_err {
   IDX2G_CHUNK_NOT_FOUND = 35,
   IDX2G_BRICK_NOT_FOUND = 36,
   IDX2G_FILE_NOT_FOUND = 37,
+  IDX2G_UNSUPPORTED_SCHEME = 38,
   IDX2G_DEVICE_ERROR = 100, /* HIP runtime failure; see idx2g_last_error_message */
 };
 
@@ -200,7 +220,8 @@ int idx2g_get_tuning(int hip_device, idx2g_tuning* t);
  * HBM footprint fits `hbm_budget_bytes` (a request that exceeds the budget
  * alone runs when nothing else holds memory).  Requests run concurrently;
  * each result is the same as idx2g_decode's for the same query.  Datasets
- * must outlive the server; they are only read. */
+ * are only read; idx2g_dataset_free on a dataset with requests queued or
+ * running returns at once and the last of those requests releases it. */
 typedef struct idx2g_server idx2g_server;
 int idx2g_server_create(int hip_device, int64_t hbm_budget_bytes, int slots, idx2g_server** out);
 /* Queue a decode of q (NULL: whole volume) into out (idx2g_output_dims

@@ -4,8 +4,9 @@
  * cpu_baseline leg of bench.py may load this code, and only as the checker.
  * The product (idx2_amd, libidx2g) never links or calls it.
  *
- * Error codes follow the reference enum numbering (idx2Common.h:36-56,
- * ErrorCodes.h:6-12).
+ * Error codes are the values of the reference enum class idx2_err_code
+ * (idx2Common.h:36-56 after ErrorCodes.h:6-12; Enum.h:65-80 puts __Invalid__
+ * last, so NoError = 0).
  */
 #ifndef IDX2_ORACLE_H
 #define IDX2_ORACLE_H
@@ -17,7 +18,7 @@ extern "C" {
 
 enum {
   IDX2O_OK = 0,
-  IDX2O_SIZE_MISMATCHED = 5,
+  IDX2O_SIZE_MISMATCHED = 4,
   IDX2O_BRICK_SIZE_NOT_POW2 = 21,
   IDX2O_BRICK_SIZE_TOO_BIG = 22,
   IDX2O_TOO_MANY_LEVELS = 23,
