@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
     // data region into this level's grid (read by the next finer level)
     if (a.cgrid) {
       int ext[3];
-      for (int d = 0; d < 3; ++d) ext[d] = max(0, min(a.brick[d], a.cgdims[d] - b3[d] * a.brick[d]));
+      for (int d = 0; d < 3; ++d) ext[d] = max(0, min(a.brick[d], a.cgend[d] - b3[d] * a.brick[d]));
       const int n = ext[0] * ext[1] * ext[2];
       for (int i = tid; i < n; i += blockDim.x) {
         const int x = i % ext[0], y = (i / ext[0]) % ext[1], z = i / (ext[0] * ext[1]);

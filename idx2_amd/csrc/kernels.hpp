@@ -138,10 +138,14 @@ struct DecArgs {
   int lds_brick;         // generic kernel: byte offset of the extended brick in LDS, -1: in scratch
   u64* xs;               // b32: plane words of the batch [brick][subband][128 blocks][npl]
   uint8_t* nbps;         // b32: planes decoded per block [brick][subband][128]
+  // Level grids hold only the query box's bricks: pgrid / cgrid are shifted
+  // base pointers, so that (Z * dims[1] + Y) * dims[0] + X with the level
+  // grid's global coordinates lands inside the box (dims: the box's extent).
   const double* pgrid;   // level+1 decoded data (dense f64; subband-0 interiors of this level)
   int pgdims[3];
   double* cgrid;         // this level's decoded data when it is not the output level
   int cgdims[3];
+  int cgend[3];          // global end (exclusive) of the cgrid box per axis
   int sbne[3];
   const uint32_t* blocks;
   int blk_off[8], blk_n[8], ncoded[8];
