@@ -67,7 +67,7 @@ EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_e
            "idx2g_synth_field", "idx2g_server_create", "idx2g_server_submit", "idx2g_server_wait",
            "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint", "idx2g_encode_batch",
            "idx2g_decode_batch", "idx2g_validate_params", "idx2g_autotune", "idx2g_set_tuning",
-           "idx2g_get_tuning"]
+           "idx2g_get_tuning", "idx2g_decode_noalloc"]
 
 _lib = None
 
@@ -103,6 +103,7 @@ def lib():
                                          ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]
         L.idx2g_output_dims.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.idx2g_decode.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.c_int]
+        L.idx2g_decode_noalloc.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.POINTER(i64)]
         L.idx2g_last_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.idx2g_dataset_chunk_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(i64)]
         L.idx2g_synth_field.argtypes = [vp, vp, ctypes.POINTER(i32), i32, ctypes.POINTER(ctypes.c_double),

@@ -95,7 +95,56 @@ size_t round_bytes(size_t n) {
 }
 }  // namespace
 
+namespace {
+thread_local ScratchArena* tl_arena = nullptr;
+
+hipError_t arena_alloc(ScratchArena& A, void** p, size_t bytes) {
+  const size_t n = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+  for (auto it = A.free_.begin(); it != A.free_.end(); ++it) {
+    if (it->second < n) continue;
+    const size_t off = it->first, left = it->second - n;
+    A.free_.erase(it);
+    if (left) A.free_[off + n] = left;
+    A.used[off] = n;
+    A.in_use += n;
+    A.peak = std::max(A.peak, A.in_use);
+    *p = A.base + off;
+    return hipSuccess;
+  }
+  *p = nullptr;
+  A.exhausted = true;
+  return hipErrorOutOfMemory;
+}
+
+bool arena_free(ScratchArena& A, void* p) {
+  const uint8_t* q = static_cast<const uint8_t*>(p);
+  if (q < A.base || q >= A.base + A.cap) return false;
+  auto it = A.used.find((size_t)(q - A.base));
+  if (it == A.used.end()) return true;
+  size_t off = it->first, n = it->second;
+  A.used.erase(it);
+  A.in_use -= n;
+  auto nx = A.free_.lower_bound(off);
+  if (nx != A.free_.end() && nx->first == off + n) {  // merge with the next free block
+    n += nx->second;
+    nx = A.free_.erase(nx);
+  }
+  if (nx != A.free_.begin()) {  // and with the previous one
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == off) {
+      pv->second += n;
+      return true;
+    }
+  }
+  A.free_[off] = n;
+  return true;
+}
+}  // namespace
+
+void set_thread_arena(ScratchArena* a) { tl_arena = a; }
+
 hipError_t dev_alloc(void** p, size_t bytes) {
+  if (tl_arena) return arena_alloc(*tl_arena, p, bytes);
   DevCache& c = dev_cache();
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -128,6 +177,7 @@ hipError_t dev_alloc(void** p, size_t bytes) {
 
 void dev_free(void* p) {
   if (!p) return;
+  if (tl_arena && arena_free(*tl_arena, p)) return;
   DevCache& c = dev_cache();
   std::lock_guard<std::mutex> g(c.mu);
   auto it = c.live.find(p);

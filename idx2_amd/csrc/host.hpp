@@ -2,6 +2,7 @@
 #pragma once
 
 #include <atomic>
+#include <map>
 
 #include <hip/hip_runtime.h>
 
@@ -33,6 +34,11 @@ struct Context;
 
 }  // namespace idx2g
 
+namespace idx2g {
+// pinned launch-table staging a context allocates when it is created
+constexpr size_t kStageInitBytes = size_t(4) << 20;
+}
+
 struct idx2g_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -44,6 +50,10 @@ struct idx2g_ctx {
   uint8_t* h_stage = nullptr;
   size_t stage_cap = 0;
   bool stage_pending = false;
+  // set while idx2g_decode_noalloc runs: device buffers come from the
+  // device's fixed scratch array, uploads must fit the staging buffer as it
+  // is, launch shapes are capped to fit the array
+  bool noalloc = false;
 };
 
 struct idx2g_dataset {
@@ -67,6 +77,27 @@ int hip_fail(hipError_t e, const char* what, int line);
 
 // Launch shapes of the current device (set by the autotuner or the caller).
 idx2g_tuning tuning_of(int device);
+
+// Device buffers of the calling thread carved from a fixed device region
+// instead of hipMalloc (the no-allocation decode): first fit in 256-byte
+// units, blocks handed back by dev_free and merged with free neighbours;
+// when nothing fits, dev_alloc returns hipErrorOutOfMemory.
+struct ScratchArena {
+  uint8_t* base = nullptr;
+  size_t cap = 0, in_use = 0, peak = 0;
+  bool exhausted = false;          // an allocation found no room
+  std::map<size_t, size_t> free_;  // offset -> bytes
+  std::map<size_t, size_t> used;   // offset -> bytes
+  void reset() {
+    free_.clear();
+    used.clear();
+    if (cap) free_[0] = cap;
+    in_use = peak = 0;
+    exhausted = false;
+  }
+};
+// While set (non-null), dev_alloc / dev_free of this thread use the arena.
+void set_thread_arena(ScratchArena* a);
 
 // Device memory through a per-device cache of freed blocks: repeated
 // encode/decode calls reuse their large buffers instead of paying

@@ -110,9 +110,14 @@ static bool full_2d32(const Layout& L, const int* b3, const int gd[3]) {
 
 }  // namespace idx2g
 
+namespace idx2g {
+static int noalloc_setup(int device);  // noalloc.inc
+}
+
 #include "encode.inc"
 #include "files.inc"
 #include "decode.inc"
 #include "api.inc"
 #include "server.inc"
 #include "tune.inc"
+#include "noalloc.inc"

@@ -172,6 +172,24 @@ int idx2g_output_dims(const idx2g_dataset* ds, const idx2g_query* q, int64_t dim
 int idx2g_decode(idx2g_ctx* ctx, idx2g_dataset* ds, const idx2g_query* q, void* out,
                  int out_is_device);
 
+/* Decode without device allocations.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * idx2g_decode of the same query, into a caller-allocated device buffer of
+ * idx2g_output_dims bytes.  Every buffer the decode needs beyond the dataset
+ * and the output comes from a fixed 128 MiB array per GPU in the library's
+ * code object (held until the process ends) and from the context's launch-
+ * table staging (4 MiB of pinned host memory made by idx2g_create); the call
+ * makes no hipMalloc/hipFree/hipHostMalloc.  A query whose working set under
+ * this call's launch shapes (at most 64 full bricks per walk batch, 64
+ * resident workgroups) exceeds the array returns IDX2G_OUT_OF_MEMORY before
+ * anything runs.  All calls on one GPU share the array and would race on it
+ * when they overlap; the library runs them one at a time (a per-device lock;
+ * each call returns after its kernels have finished).  scratch_bytes_used
+ * (may be NULL): the peak bytes of the array this call used. */
+int idx2g_decode_noalloc(idx2g_ctx* ctx, idx2g_dataset* ds, const idx2g_query* q, void* out_device,
+                         int64_t* scratch_bytes_used);
+
 /* Batched decode: no counterpart in the reference (one query per Decode
  * call).  Query i (queries NULL: whole volumes) of dataset ds[i] into out[i];
  * each level's bricks of all queries run in shared launches.  Every result
