@@ -12,11 +12,11 @@
 //      transpose across the wave that leaves one bit plane per lane;
 //   3. per bit plane: significance count N by a suffix-max scan over the
 //      planes above it, emitted length by closed form (no bit loop);
-//   4. per key (bit-plane channel): exclusive scan of the lengths over the
-//      blocks in Morton order, one wave per key;
-//   5. one thread per (block, plane) emits its bits into an LDS stream
-//      buffer (global arena when the streams do not fit), then the buffer is
-//      copied into the arena.
+//   4. per key (bit-plane channel), one wave per key: an exclusive scan of
+//      the plane lengths over the blocks in Morton order gives each plane's
+//      bit offset in the stream, and the same lane emits the plane's bits
+//      there, into an LDS stream buffer (the global arena when the streams
+//      do not fit), which is then copied into the arena.
 #pragma once
 
 #include "brick32.hpp"
@@ -49,8 +49,8 @@ struct CodeLds {
     uint32_t stream[kStreamWords];
   } v;
   i64 T[kTh / 64][4][64];         // row-mode transpose buffers, per wave
-  // per (block, plane): len | N << 8 | bit offset << 15, stride npl; for
-  // f32 the plane words follow (f64 plane words go to global scratch)
+  // per (block, plane): len | N << 8, stride npl; for f32 the plane words
+  // follow (f64 plane words go to global scratch)
   uint32_t pool[16384];
   short E[128];                   // block exponent (kSkip: not coded)
   unsigned char np[128];          // coded planes of the block
@@ -426,15 +426,17 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
     const bool rowok = !skip && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
     const int n0 = nd >= 1 ? 4 : 1;
     const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
-    double v[4];
-    int emx = -4000;
+    // common exponent as QuantizeF32/F64 (Function.h:103-105): the largest
+    // |value| first (per row, then over the block's rows), one exponent per
+    // row; padding entries are 0 and do not change the maximum
+    double v[4], m = 0.0;
 #pragma unroll
     for (int i0 = 0; i0 < 4; ++i0) {
       const bool ok = rowok && i0 < n0;
       v[i0] = ok ? C.v.V[vbase + i0 * s0] : 0.0;
-      const int ev = DT ? exponent_f64(fabs(v[i0])) : exponent_f32((float)fabs(v[i0]));
-      if (ok) emx = max(emx, ev);
+      m = fmax(m, fabs(v[i0]));
     }
+    int emx = rowok ? (DT ? exponent_f64(m) : exponent_f32((float)m)) : -4000;
     emx = row_max(emx);
     const int emax = emx;
     const double scale = pow2d((63 - nd) - 1 - emax);
@@ -568,8 +570,8 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     return;
   }
   // the block stage has summed each key stream's bits; wave 0 lays out the
-  // streams and reserves their arena slice while the other waves compute each
-  // (block, plane)'s bit offset inside its stream
+  // streams and reserves their arena slice while every wave clears the LDS
+  // stream buffer (it aliases V, free since the block stage)
   constexpr int M = kMaxKeys - 1;
   if (p.w == 0) {
     uint32_t carry = 0;
@@ -610,22 +612,10 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
       h.ebase = ebs;
       gm_store(a.hdrs + (size_t)dense * k.nsb + sb, h);
     }
-  } else {
-    for (int kk = p.w - 1; kk < nk; kk += kTh / 64 - 1) {
-      const int key = klo + kk;
-      uint32_t carry = 0;
-      for (int c0 = 0; c0 < nblk; c0 += 64) {
-        const int b = c0 + lane;
-        const int e = b < nblk ? C.E[b] : kSkip;
-        const int pp = 63 - (key - kBpBiasDev - e);
-        const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
-        const uint32_t len = coded ? (lno_at(C, npl, b, pp) & 0xff) : 0u;
-        const uint32_t incl = wave_scan_add(len, lane);
-        if (coded) lno_at(C, npl, b, pp) |= (incl - len + carry) << 15;
-        carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-      }
-    }
   }
+  static_assert(kStreamWords % (4 * kTh) == 0, "stream buffer cleared in 16-byte units");
+#pragma unroll
+  for (int i = p.tid; i < kStreamWords / 4; i += kTh) reinterpret_cast<uint4*>(C.v.stream)[i] = uint4{0, 0, 0, 0};
   __syncthreads();
   if (C.over) return;
   const unsigned long long base = C.base;
@@ -641,27 +631,39 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     e.key = (uint16_t)(klo + kk);
     gm_store(a.entries + C.ebase + kk, e);
   }
-  // ---- 5. emission ----
   uint32_t* words = staged ? C.v.stream : gwords;
-  for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
-  if (!staged) __threadfence();
-  __syncthreads();
-  // each wave emits the coded planes of its share of the blocks, packed
-  // over its lanes (no lane idles on a plane beyond a block's last)
-  const int per_w = (nblk + kTh / 64 - 1) / (kTh / 64);
-  const int wb0 = min(nblk, p.w * per_w);
-  for_coded_planes(C, wb0, min(per_w, nblk - wb0), lane, [&](int b, int pp) {
-    const int e = C.E[b];
-    const uint32_t bw = blocks[b];
-    const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
-    const uint32_t lno = lno_at(C, npl, b, pp);
-    u64 lo, hi;
-    plane_bits(plane_load(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
-    const int ring = ((63 - pp) + e + kBpBiasDev) & M;
-    const u64 pos = (u64)C.kbyte[ring] * 8 + (lno >> 15);
-    if (staged) or_bits128<true>(words, pos, lo, hi);
-    else or_bits128<false>(words, pos, lo, hi);
-  });
+  if (!staged) {  // streams larger than the LDS buffer are assembled in the arena
+    for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
+    __threadfence();
+    __syncthreads();
+  }
+  // ---- 4+5. per key: exclusive scan of the plane lengths over the blocks
+  // in Morton order gives each plane's bit offset in the key's stream, and
+  // the same lane emits the plane's bits there ----
+  for (int kk = p.w; kk < nk; kk += kTh / 64) {
+    const int key = klo + kk;
+    const u64 sbit = (u64)C.kbyte[key & M] * 8;
+    uint32_t carry = 0;
+    for (int c0 = 0; c0 < nblk; c0 += 64) {
+      const int b = c0 + lane;
+      const int e = b < nblk ? C.E[b] : kSkip;
+      const int pp = 63 - (key - kBpBiasDev - e);
+      const bool coded = b < nblk && e != kSkip && pp >= 0 && pp < C.np[b];
+      const uint32_t lno = coded ? lno_at(C, npl, b, pp) : 0u;
+      const uint32_t len = lno & 0xff;
+      const uint32_t incl = wave_scan_add(len, lane);
+      if (coded) {
+        const uint32_t bw = blocks[b];
+        const int nd = (((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1);
+        u64 lo, hi;
+        plane_bits(plane_load(C, XG, npl, b, pp), int((lno >> 8) & 0x7f), 1 << (2 * nd), C.e8, lo, hi);
+        const u64 pos = sbit + incl - len + carry;
+        if (staged) or_bits128<true>(words, pos, lo, hi);
+        else or_bits128<false>(words, pos, lo, hi);
+      }
+      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+  }
   __syncthreads();
   if (staged)
     for (uint32_t i = p.tid; i < nwords; i += kTh) gm(gwords)[i] = words[i];

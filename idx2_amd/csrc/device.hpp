@@ -315,12 +315,15 @@ __device__ __forceinline__ double pow2d(int e) {
 }
 // (i64)x for |x| < 2^63: truncation toward zero, split exactly into 32-bit
 // halves (t and hi*2^32 are exact, so lo is exact).
+// NaN and |x| >= 2^63 give 0x8000000000000000 (x86's cvttsd2si); computed
+// without a branch (the hardware conversions saturate, and the select drops
+// their result for such x).
 __device__ __forceinline__ i64 f64_to_i64(double x) {
-  if (!(x == x) || fabs(x) >= 9.2233720368547758e18) return (i64)0x8000000000000000ll;
-  double t = trunc(x);
-  double hi = floor(t * 2.3283064365386963e-10);
-  double lo = t - hi * 4294967296.0;
-  return (i64)(((u64)(i64)(int)hi << 32) | (u64)(u32)lo);
+  const double t = trunc(x);
+  const double hi = floor(t * 2.3283064365386963e-10);
+  const double lo = t - hi * 4294967296.0;
+  const i64 r = (i64)(((u64)(u32)__double2int_rz(hi) << 32) | (u64)__double2uint_rz(lo));
+  return fabs(x) < 9.2233720368547758e18 ? r : (i64)0x8000000000000000ll;
 }
 // (double)q, correctly rounded: hi*2^32 is exact and the sum rounds once.
 __device__ __forceinline__ double i64_to_f64(i64 q) {
