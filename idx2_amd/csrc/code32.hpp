@@ -25,7 +25,10 @@
 namespace idx2g {
 namespace b32 {
 
-constexpr int kStreamWords = 4096;  // LDS stream buffer (16 KB), aliases V
+constexpr int kStreamWords = 4096;  // LDS stream buffer (16 KB)
+// per (block, plane) words of the pool: the length word of every plane plus,
+// for float32, the plane word (two 32-bit words) after them
+constexpr int kPoolWords = kMaxBlocks * 32 * 3;
 constexpr unsigned long long kArenaChunk = 256ull << 10;  // arena bytes a workgroup takes at a time
 constexpr unsigned long long kEntryChunk = 2048;          // stream entries a workgroup takes at a time
 
@@ -44,14 +47,17 @@ __device__ __forceinline__ void flush_cursor(const ArenaCursor& cur, unsigned lo
 }
 
 struct CodeLds {
-  union {
+  struct {
     double V[17 * 17 * 17];
-    uint32_t stream[kStreamWords];
   } v;
+  // the streams of one subband; all zero between subbands (the copy into
+  // the arena clears what it copies), so the next subband's staging into V
+  // does not wait for that copy
+  uint32_t stream[kStreamWords];
   i64 T[kTh / 64][4][64];         // row-mode transpose buffers, per wave
   // per (block, plane): len | N << 8, stride npl; for f32 the plane words
   // follow (f64 plane words go to global scratch)
-  uint32_t pool[16384];
+  uint32_t pool[kPoolWords];
   short E[128];                   // block exponent (kSkip: not coded)
   unsigned char np[128];          // coded planes of the block
   // per key stream, indexed by key % kMaxKeys (a subband's keys span fewer
@@ -570,8 +576,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     return;
   }
   // the block stage has summed each key stream's bits; wave 0 lays out the
-  // streams and reserves their arena slice while every wave clears the LDS
-  // stream buffer (it aliases V, free since the block stage)
+  // streams and reserves their arena slice
   constexpr int M = kMaxKeys - 1;
   if (p.w == 0) {
     uint32_t carry = 0;
@@ -613,9 +618,6 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
       gm_store(a.hdrs + (size_t)dense * k.nsb + sb, h);
     }
   }
-  static_assert(kStreamWords % (4 * kTh) == 0, "stream buffer cleared in 16-byte units");
-#pragma unroll
-  for (int i = p.tid; i < kStreamWords / 4; i += kTh) reinterpret_cast<uint4*>(C.v.stream)[i] = uint4{0, 0, 0, 0};
   __syncthreads();
   if (C.over) return;
   const unsigned long long base = C.base;
@@ -631,7 +633,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     e.key = (uint16_t)(klo + kk);
     gm_store(a.entries + C.ebase + kk, e);
   }
-  uint32_t* words = staged ? C.v.stream : gwords;
+  uint32_t* words = staged ? C.stream : gwords;
   if (!staged) {  // streams larger than the LDS buffer are assembled in the arena
     for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
     __threadfence();
@@ -665,9 +667,14 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     }
   }
   __syncthreads();
+  // into the arena, leaving the LDS buffer zero for the next subband (its
+  // emission comes after at least two more barriers)
   if (staged)
-    for (uint32_t i = p.tid; i < nwords; i += kTh) gm(gwords)[i] = words[i];
-  __syncthreads();
+    for (uint32_t i = p.tid; i < nwords; i += kTh) {
+      const uint32_t w = words[i];
+      words[i] = 0;
+      gm(gwords)[i] = w;
+    }
 }
 
 template <int DT>
@@ -685,6 +692,10 @@ __device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[k
     }
     C.e8[p.tid] = (uint16_t)e;
   }
+  // the stream buffer shares LDS with the transform's row buffers: clear it
+  // once per brick (the first subband's barriers order this before use)
+  static_assert(kStreamWords % (4 * kTh) == 0, "stream buffer cleared in 16-byte units");
+  for (int i = p.tid; i < kStreamWords / 4; i += kTh) reinterpret_cast<uint4*>(C.stream)[i] = uint4{0, 0, 0, 0};
   // the z-odd half leaves the registers first; the z-even subbands are coded
   // from the registers, the z-odd ones from the parked copy
   double* Z1 = reinterpret_cast<double*>(XG) + kOddOff;
