@@ -61,16 +61,17 @@ struct CodeLds {
   short E[128];                   // block exponent (kSkip: not coded)
   unsigned char np[128];          // coded planes of the block
   // per key stream, indexed by key % kMaxKeys (a subband's keys span fewer
-  // than kMaxKeys): bits, byte offset, first block with a plane in it
+  // than kMaxKeys): bits, first block with a plane in it
   uint32_t kbits[kMaxKeys];
-  uint32_t kbyte[kMaxKeys];
   int kfirst[kMaxKeys];
   unsigned char p3[64], p2[16];
   uint16_t e8[256];               // byte -> bits with a 1 inserted after every 1
   int klo, khi;
-  unsigned long long base, ebase;
-  uint32_t total;
-  int over;
+  // arena slice and overflow flag of the subband with sequence number q in
+  // slot q & 1: read after the emission's barrier, when the next subband may
+  // already write its own slot
+  unsigned long long base_s[2];
+  int over_s[2];
 };
 
 // ---- wave helpers ----
@@ -546,10 +547,11 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
 // Code every subband of the brick held in c (register columns) and S
 // (faces); writes exponent rows, streams into the arena, entries, headers.
 // One subband of the brick: stage into V, block stage, key offsets, emission.
+// seq: the subband's position in the brick's coding order.
 template <int DT, bool kOdd>
-__device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const double (&c)[kN], const double* Z1, Lds& S,
-                                             CodeLds& C, const Pos& p, int dense, u64* XG, const uint32_t* s_blocks,
-                                             ArenaCursor& cur) {
+__device__ __forceinline__ void code_subband(const EncArgs& a, int sb, int seq, const double (&c)[kN], const double* Z1,
+                                             Lds& S, CodeLds& C, const Pos& p, int dense, u64* XG,
+                                             const uint32_t* s_blocks, ArenaCursor& cur) {
   const BrickConst& k = a.k;
   constexpr int npl = DT ? 64 : 32;
   const int lane = p.lane;
@@ -559,7 +561,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   const uint32_t* blocks = s_blocks + sb * 128;
   if (kOdd) stage_subband_odd(Z1, S, C.v.V, p, hx, hy);
   else stage_subband_even(c, S, C.v.V, p, hx, hy);
-  if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; C.over = 0; }
+  if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; }
   if (p.tid < kMaxKeys) { C.kbits[p.tid] = 0; C.kfirst[p.tid] = 0x7fffffff; }
   __syncthreads();
   block_stage<DT>(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
@@ -575,20 +577,26 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     __syncthreads();
     return;
   }
-  // the block stage has summed each key stream's bits; wave 0 lays out the
-  // streams and reserves their arena slice
+  // the block stage has summed each key stream's bits: every wave lays the
+  // streams out itself (an exclusive scan of at most kMaxKeys byte sizes),
+  // so that the emission starts at once; wave 0 also reserves the arena
+  // slice and writes the header, which only the copy below needs
   constexpr int M = kMaxKeys - 1;
+  static_assert(kMaxKeys == 128, "keys scanned in two 64-lane chunks");
+  const uint32_t by0 = lane < nk ? (C.kbits[(klo + lane) & M] + 7) >> 3 : 0u;
+  const uint32_t by1 = lane + 64 < nk ? (C.kbits[(klo + lane + 64) & M] + 7) >> 3 : 0u;
+  const uint32_t in0 = wave_scan_add(by0, lane);
+  const uint32_t in1 = wave_scan_add(by1, lane) + (uint32_t)__builtin_amdgcn_readlane((int)in0, 63);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)in1, 63);
+  const uint32_t ex0 = in0 - by0, ex1 = in1 - by1;  // byte offsets of keys klo + lane, klo + lane + 64
+  const uint32_t nwords = (total + 3) >> 2;
+  const bool staged = nwords <= (uint32_t)kStreamWords;
+  const int slot = seq & 1;
   if (p.w == 0) {
-    uint32_t carry = 0;
-    for (int c0 = 0; c0 < nk; c0 += 64) {
-      const int kk = c0 + lane;
-      const uint32_t by = kk < nk ? (C.kbits[(klo + kk) & M] + 7) >> 3 : 0u;
-      const uint32_t incl = wave_scan_add(by, lane);
-      if (kk < nk) C.kbyte[(klo + kk) & M] = incl - by + carry;
-      carry += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-    }
+    unsigned long long bse = 0, ebs = 0;
+    int over = 0;
     if (lane == 0) {
-      const unsigned long long alloc = (carry + 3ull) & ~3ull;
+      const unsigned long long alloc = (total + 3ull) & ~3ull;
       if (alloc > cur.aleft) {
         const unsigned long long take = alloc > kArenaChunk ? alloc : kArenaChunk;
         cur.abase = atomicAdd(&a.counters[0], take);
@@ -599,42 +607,48 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
         cur.ebase = atomicAdd(&a.counters[1], take);
         cur.eleft = take;
       }
-      const unsigned long long bse = cur.abase, ebs = cur.ebase;
+      bse = cur.abase;
+      ebs = cur.ebase;
       cur.abase += alloc;
       cur.aleft -= alloc;
       cur.ebase += nk;
       cur.eleft -= nk;
-      cur.bytes += carry;
-      const bool over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
+      cur.bytes += total;
+      over = bse + alloc > a.arena_cap || ebs + nk > a.entry_cap;
       if (over) atomicOr(&a.counters[2], 2ull);
-      C.base = bse;
-      C.ebase = ebs;
-      C.total = carry;
-      C.over = over;
+      C.base_s[slot] = bse;
+      C.over_s[slot] = over;
       SbHdr h;
       h.klo = klo;
       h.nkeys = over ? 0 : nk;
       h.ebase = ebs;
       gm_store(a.hdrs + (size_t)dense * k.nsb + sb, h);
     }
+    // the stream entries, from lane 0's reservation
+    bse = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(bse >> 32), 0) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)bse, 0);
+    ebs = ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(ebs >> 32), 0) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ebs, 0);
+    over = __builtin_amdgcn_readlane(over, 0);
+    if (!over)
+#pragma unroll
+      for (int h2 = 0; h2 < 2; ++h2) {
+        const int kk = lane + 64 * h2;
+        if (kk >= nk) continue;
+        const int ring = (klo + kk) & M;
+        SbEntry e;
+        e.off = bse + (h2 ? ex1 : ex0);
+        e.size = (C.kbits[ring] + 7) >> 3;
+        e.first_block = (uint16_t)(C.kfirst[ring] == 0x7fffffff ? 0xffff : C.kfirst[ring]);
+        e.key = (uint16_t)(klo + kk);
+        gm_store(a.entries + ebs + kk, e);
+      }
   }
-  __syncthreads();
-  if (C.over) return;
-  const unsigned long long base = C.base;
-  const uint32_t nwords = (C.total + 3) >> 2;
-  const bool staged = nwords <= (uint32_t)kStreamWords;
-  uint32_t* gwords = (uint32_t*)(a.arena + base);
-  for (int kk = p.tid; kk < nk; kk += kTh) {
-    const int ring = (klo + kk) & M;
-    SbEntry e;
-    e.off = base + C.kbyte[ring];
-    e.size = (C.kbits[ring] + 7) >> 3;
-    e.first_block = (uint16_t)(C.kfirst[ring] == 0x7fffffff ? 0xffff : C.kfirst[ring]);
-    e.key = (uint16_t)(klo + kk);
-    gm_store(a.entries + C.ebase + kk, e);
-  }
-  uint32_t* words = staged ? C.stream : gwords;
+  uint32_t* words = C.stream;
   if (!staged) {  // streams larger than the LDS buffer are assembled in the arena
+    __syncthreads();
+    if (C.over_s[slot]) return;
+    words = (uint32_t*)(a.arena + C.base_s[slot]);
     for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
     __threadfence();
     __syncthreads();
@@ -644,7 +658,9 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
   // the same lane emits the plane's bits there ----
   for (int kk = p.w; kk < nk; kk += kTh / 64) {
     const int key = klo + kk;
-    const u64 sbit = (u64)C.kbyte[key & M] * 8;
+    const int ku = __builtin_amdgcn_readfirstlane(kk);  // the wave's key (uniform)
+    const uint32_t kbyte = (uint32_t)__builtin_amdgcn_readlane((int)(ku < 64 ? ex0 : ex1), ku & 63);
+    const u64 sbit = (u64)kbyte * 8;
     uint32_t carry = 0;
     for (int c0 = 0; c0 < nblk; c0 += 64) {
       const int b = c0 + lane;
@@ -667,6 +683,12 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, const dou
     }
   }
   __syncthreads();
+  if (C.over_s[slot]) {  // no room in the arena (reported through the counters): leave the buffer zero
+    if (staged)
+      for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;
+    return;
+  }
+  uint32_t* gwords = (uint32_t*)(a.arena + C.base_s[slot]);
   // into the arena, leaving the LDS buffer zero for the next subband (its
   // emission comes after at least two more barriers)
   if (staged)
@@ -700,10 +722,11 @@ __device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[k
   // from the registers, the z-odd ones from the parked copy
   double* Z1 = reinterpret_cast<double*>(XG) + kOddOff;
   park_odd(c, Z1, p);
+  int seq = 0;
   for (int sb = 0; sb < 8; ++sb)
-    if (k.sb[sb].f[2] == 0) code_subband<DT, false>(a, sb, c, Z1, S, C, p, dense, XG, s_blocks, cur);
+    if (k.sb[sb].f[2] == 0) code_subband<DT, false>(a, sb, seq++, c, Z1, S, C, p, dense, XG, s_blocks, cur);
   for (int sb = 0; sb < 8; ++sb)
-    if (k.sb[sb].f[2] == 1) code_subband<DT, true>(a, sb, c, Z1, S, C, p, dense, XG, s_blocks, cur);
+    if (k.sb[sb].f[2] == 1) code_subband<DT, true>(a, sb, seq++, c, Z1, S, C, p, dense, XG, s_blocks, cur);
 }
 
 }  // namespace b32
