@@ -32,12 +32,12 @@ CONFIGS = {
     "c1": dict(dims=(384, 384, 256), dtype="float64", nlevels=2, tolerance=1e-3, brick=(32, 32, 32),
                workload="3D float64 384x384x256, brick 32^3, 2 levels, tolerance 1e-3"),
 }
-# bounded CPU sample of the same workload for the reference binary
-CPU_SAMPLE = {
-    "c2": dict(dims=(512, 512, 1024), dtype="float32", nlevels=3, tolerance=1e-4, brick=(32, 32, 32)),
-    "c2b": dict(dims=(2048, 2048, 1), dtype="float64", nlevels=3, tolerance=1e-4, brick=(32, 32, 1)),
-    "c1": dict(dims=(384, 384, 256), dtype="float64", nlevels=2, tolerance=1e-3, brick=(32, 32, 32)),
-}
+# The CPU baseline runs the reference's own CPU configuration (the 384x384x256
+# float64 case, 2 levels, tolerance 1e-3) whatever GPU workload is measured:
+# one core for encode and serial decode, and the reference's thread-pool
+# decoder on at most CPU_SHARE cores (a one-GPU job's share of the host).
+CPU_SAMPLE = CONFIGS["c1"]
+CPU_SHARE = 16
 
 
 def log(msg):
@@ -45,48 +45,81 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of the dominant kernel from its committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, each calibrated against a plain device copy); None if absent."""
-    name = kernel.split()[0].split("::")[-1]
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{name}_1024cube_f32.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
-            return int(line.split(":")[1])
-    return None
-
-
-def cpu_baseline(ctx, torch, cfg_name, seed):
-    """Time the reference idx2App (built from the reference sources into
-    oracle/_ref) on a bounded sample of the same workload: encode + serial
-    full decode, one core."""
+def cpu_baseline(ctx, torch, seed):
+    """The reference idx2App (built from the reference sources into
+    oracle/_ref) on its CPU configuration: encode + serial full decode pinned
+    to one core (the value), and the --parallel decode pinned to CPU_SHARE
+    cores.  The raw input file is written before the clock starts; file
+    output of the encode and decode is included (it is part of idx2App)."""
     from oracle import oracle as O  # checker/baseline only
     if not O.ref_available():
         return None
-    s = CPU_SAMPLE[cfg_name]
+    s = CPU_SAMPLE
     nx, ny, nz = s["dims"]
     tdt = torch.float64 if s["dtype"] == "float64" else torch.float32
     buf = torch.empty(nx * ny * nz, dtype=tdt, device="cuda")
     ctx.synth_field(buf.data_ptr(), s["dims"], s["dtype"], seed)
     torch.cuda.synchronize()
     host = buf.cpu().numpy().reshape(nz, ny, nx)
-    raw = host.nbytes
+    raw_bytes = host.nbytes
+    avail = sorted(os.sched_getaffinity(0))
+    one, share = avail[:1], avail[:min(CPU_SHARE, len(avail))]
     with tempfile.TemporaryDirectory() as wd:
+        raw = O.ref_write_raw(wd, "B", "f", host)
+        out_dir = os.path.join(wd, "ref")
         t0 = time.perf_counter()
-        rc, rdir = O.ref_encode(wd, "B", "f", host, brick=s["brick"], nlevels=s["nlevels"], tolerance=s["tolerance"])
+        rc = O.ref_encode_raw(raw, out_dir, brick=s["brick"], nlevels=s["nlevels"], tolerance=s["tolerance"], cpus=one)
         t1 = time.perf_counter()
         if rc:
             return None
-        rc, _ = O.ref_decode(wd, os.path.join(rdir, "B", "f.idx2"))
+        meta = os.path.join(out_dir, "B", "f.idx2")
+        rc, _ = O.ref_decode(wd, meta, cpus=one)
         t2 = time.perf_counter()
-        if rc:
+        rc2, _ = O.ref_decode(wd, meta, parallel=True, cpus=share)
+        t3 = time.perf_counter()
+        if rc or rc2:
             return None
-    return {"value": round(raw / ((t2 - t0)) / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": "reference",
-            "sample": f"{nx}x{ny}x{nz} {s['dtype']} of the same generator, brick {s['brick'][0]}, "
-                      f"{s['nlevels']} levels, tol {s['tolerance']}: reference idx2App encode "
-                      f"{t1 - t0:.2f}s + serial decode {t2 - t1:.2f}s (includes file I/O)"}
+    return {"value": round(raw_bytes / (t2 - t0) / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": "reference",
+            "sample": f"{s['workload']}: reference idx2App encode {t1 - t0:.2f}s + serial decode {t2 - t1:.2f}s "
+                      f"on one core (file output included, raw input written before timing)",
+            "parallel_decode": {"seconds": round(t3 - t2, 3), "gbps": round(raw_bytes / (t3 - t2) / 1e9, 6),
+                                "cores": len(share),
+                                "note": "idx2App --decode --parallel (thread pool of hardware_concurrency() threads) "
+                                        "pinned to the listed number of cores"}}
+
+
+# FP64 operations of the reference's transform of one full 32^3 brick,
+# counted from its loops (FLiftCdf53 / ILiftCdf53, Wavelet.h:177-310): a
+# line of D points has P = len(range(1, D - 2, 2)) interior predicts (3 ops:
+# add, halve, subtract) and as many updates (4 ops: two quarter-and-add),
+# plus for odd D one more predict (3) and update (4), for even D the
+# extrapolated point 2B - A (2).  ExtrapolateCdf53 (Wavelet.cpp:67-77) lifts
+# X, Y, Z of the 32-point data (X over 32x32 lines, Y over 33x32, Z over
+# 33x33) and the 17^3, 9^3, 5^3, 3^3 subgrids forward, then the same
+# subgrids and 33^3 inverse; the subband transform lifts 33^3 once more
+# (Wavelet.cpp:80-138) and weighs all but subband 0 (17^3).
+def fp64_ops_full_brick():
+    def line(d):
+        n = len(range(1, d - 2, 2))
+        return 3 * n + 4 * n + (7 if d % 2 else 2)
+    ext0 = 32 * 32 * line(32) + 33 * 32 * line(32) + 33 * 33 * line(32)
+    sub = sum(3 * n * n * line(n) for n in (17, 9, 5, 3))
+    full33 = 3 * 33 * 33 * line(33)
+    weights = 33 ** 3 - 17 ** 3
+    return ext0 + 2 * sub + full33 + full33 + weights
+
+
+def profile_counter(kernel, key):
+    """A per-launch figure (e.g. the rocprofv3 VALU instruction count) from the
+    committed profile summary of the dominant kernel; None if absent."""
+    name = kernel.split()[0].split("::")[-1]
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{name}_1024cube_f32.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return int(line.split(":")[1])
+    return None
 
 
 def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=48 << 30):
@@ -344,25 +377,58 @@ def plain_bench(ctx, torch, cfg, steps, warmup, seed):
     del inp, out
     enc_ms = np.array([e["total_ms"] for e in enc])
     dec_ms = np.array([d["total_ms"] for d in dec])
-    # dominant kernel: the level-0 brick launches of encode or decode, whichever is longer
+    # the level-0 brick launches: encode's brick kernel, decode's walk + on-chip pair
     brick = tuple(cfg["brick"])
     kind = "b32" if brick == (32, 32, 32) and nz > 1 else "2d" if brick == (32, 32, 1) and nz == 1 else None
     dom_e = float(np.mean([e["dominant_ms"] for e in enc]))
     dom_d = float(np.mean([d["dominant_ms"] for d in dec]))
-    if dom_e >= dom_d:
-        name = f"idx2g::k_encode_{kind} (level 0)" if kind else "idx2g::k_encode_level (level 0)"
-        dom_ms, dom_bytes, launches = dom_e, float(np.mean([e["dominant_bytes"] for e in enc])), 1
-    else:
-        name = (f"idx2g::k_walk + idx2g::k_decode_{kind} (level 0)" if kind
-                else "idx2g::k_decode_level (level 0)")
-        dom_ms, dom_bytes = dom_d, float(np.mean([d["dominant_bytes"] for d in dec]))
-        launches = int(np.mean([d["dominant_launches"] for d in dec]))
+    enc_name = f"idx2g::k_encode_{kind} (level 0)" if kind else "idx2g::k_encode_level (level 0)"
+    dec_name = f"idx2g::k_walk + idx2g::k_decode_{kind} (level 0)" if kind else "idx2g::k_decode_level (level 0)"
+    eb = float(np.mean([e["dominant_bytes"] for e in enc]))
+    db = float(np.mean([d["dominant_bytes"] for d in dec]))
     return {"value": raw * steps / wall / 1e9, "wall": wall, "enc_ms": float(enc_ms.mean()),
             "dec_ms": float(dec_ms.mean()), "enc_gbs": raw / (enc_ms.mean() / 1e3) / 1e9,
             "dec_gbs": raw / (dec_ms.mean() / 1e3) / 1e9, "dec_p50": float(np.percentile(dec_ms, 50)),
             "dec_p99": float(np.percentile(dec_ms, 99)), "chunk_bytes": chunk_bytes, "err": err,
-            "dom_name": name, "dom_ms": dom_ms, "dom_bytes": dom_bytes, "dom_launches": launches,
-            "achieved": dom_bytes / (dom_ms / 1e3) / 1e9}
+            "dom_name": enc_name, "dom_ms": dom_e, "dom_bytes": eb, "dom_launches": 1,
+            "achieved": eb / (dom_e / 1e3) / 1e9,
+            "dec_name": dec_name, "dec_dom_ms": dom_d, "dec_dom_bytes": db,
+            "dec_launches": int(np.mean([d["dominant_launches"] for d in dec])),
+            "dec_achieved": db / (dom_d / 1e3) / 1e9, "raw": raw}
+
+
+def end_to_end(ctx, torch, cfg, seed, reps=2):
+    """Encode of the HBM-resident grid followed by idx2g_dataset_write (host
+    zstd of the exponent and address sections, L*.bin and .idx2 files into a
+    tmpfs directory when the host has one): what the reference's Encode +
+    file flush delivers."""
+    from idx2_amd import api
+    nx, ny, nz = cfg["dims"]
+    tdt = torch.float64 if cfg["dtype"] == "float64" else torch.float32
+    inp = torch.empty(nx * ny * nz, dtype=tdt, device="cuda")
+    ctx.synth_field(inp.data_ptr(), cfg["dims"], cfg["dtype"], seed)
+    torch.cuda.synchronize()
+    raw = inp.numel() * inp.element_size()
+    base = "/dev/shm" if os.path.isdir("/dev/shm") else None
+    enc_s, wr_s, files = [], [], 0
+    for _ in range(reps):
+        with tempfile.TemporaryDirectory(dir=base) as wd:
+            t0 = time.perf_counter()
+            ds = api.Dataset.encode(ctx, dev_ptr=inp.data_ptr(), dims=cfg["dims"], dtype=cfg["dtype"], name="e2e",
+                                    field="f", brick=cfg["brick"], nlevels=cfg["nlevels"], tolerance=cfg["tolerance"])
+            t1 = time.perf_counter()
+            ds.write(wd)
+            t2 = time.perf_counter()
+            ds.free()
+            files = sum(os.path.getsize(os.path.join(r, f)) for r, _, fs in os.walk(wd) for f in fs)
+        enc_s.append(t1 - t0)
+        wr_s.append(t2 - t1)
+    del inp
+    e, w = min(enc_s), min(wr_s)
+    return {"encode_ms": round(e * 1e3, 2), "write_ms": round(w * 1e3, 2), "gbps": round(raw / (e + w) / 1e9, 3),
+            "file_bytes": int(files), "dir": base or tempfile.gettempdir(),
+            "note": "encode of the HBM-resident grid + idx2g_dataset_write (zstd sections and files), best of "
+                    f"{reps}"}
 
 
 def main():
@@ -385,6 +451,8 @@ def main():
                     help="skip the host-buffer (PCIe-inclusive) encode/decode")
     ap.add_argument("--no-slice", dest="slice2d", action="store_false",
                     help="skip the 2D float64 8192^2 slice")
+    ap.add_argument("--no-end-to-end", dest="end_to_end", action="store_false",
+                    help="skip the encode + file write timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration under rocprofv3) and exit")
     args = ap.parse_args()
@@ -414,7 +482,12 @@ def main():
                   "tuning_s": round(time.perf_counter() - t0, 2)}
     log(f"plain path: {cfg['workload']}")
     m = plain_bench(ctx, torch, cfg, args.steps, args.warmup, args.seed)
-    traffic = measured_traffic(m["dom_name"]) if args.config == "c2" else None
+    traffic = profile_counter(m["dom_name"], "traffic_bytes_per_launch") if args.config == "c2" else None
+    valu = profile_counter(m["dom_name"], "valu_insts_per_launch") if args.config == "c2" else None
+    bricks0 = 1
+    for d in range(3):
+        bricks0 *= cfg["dims"][d] // cfg["brick"][d]
+    fp64_ops = fp64_ops_full_brick() * bricks0 if tuple(cfg["brick"]) == (32, 32, 32) else None
     res = {
         "metric": METRIC,
         "value": round(m["value"], 3),
@@ -441,8 +514,28 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(m["achieved"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(m["achieved"] / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": m["dom_name"],
                      "kernel_ms": round(m["dom_ms"], 3), "launches": m["dom_launches"],
-                     "algorithmic_bytes": int(m["dom_bytes"])},
+                     "algorithmic_bytes": int(m["dom_bytes"]),
+                     "traffic_source": "profiles/k_encode_b32_1024cube_f32.md (rocprofv3 --pmc FETCH_SIZE / "
+                                       "WRITE_SIZE of the same launch, copied, not measured in this run)"},
+        "roofline_decode": {"bound": "hbm", "achieved": round(m["dec_achieved"], 2), "peak": HBM_PEAK_GBS,
+                            "unit": "GB/s", "frac": round(m["dec_achieved"] / HBM_PEAK_GBS, 5),
+                            "kernel": m["dec_name"], "kernel_ms": round(m["dec_dom_ms"], 3),
+                            "launches": m["dec_launches"], "algorithmic_bytes": int(m["dec_dom_bytes"])},
     }
+    if fp64_ops is not None:
+        tf = fp64_ops / (m["dom_ms"] / 1e3) / 1e12
+        res["compute"] = {"fp64": {"bound": "fp64", "ops_per_brick": fp64_ops_full_brick(), "ops": int(fp64_ops),
+                                   "achieved": round(tf, 3), "peak": 78.6, "unit": "TFLOP/s",
+                                   "frac": round(tf / 78.6, 5), "kernel": m["dom_name"],
+                                   "note": "FP64 operations of the reference's transform of the launch's bricks "
+                                           "(no FMA: the order of operations is the reference's)"}}
+        if valu is not None:
+            rate = valu / (m["dom_ms"] / 1e3) / 1e9
+            res["compute"]["valu"] = {"bound": "valu", "insts_per_launch": valu, "achieved": round(rate, 1),
+                                      "peak": 614.4, "unit": "G wave-instructions/s",
+                                      "frac": round(rate / 614.4, 4),
+                                      "source": "SQ_INSTS_VALU from profiles/k_encode_b32_1024cube_f32.md; peak = 256 "
+                                                "CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction"}
     if tuning is not None:
         res["autotune"] = tuning
     if args.slice2d and args.config == "c2":
@@ -455,6 +548,9 @@ def main():
     if args.host_path and args.config == "c2":
         log("host-buffer path")
         res["host_path"] = host_path_bench(ctx, torch, cfg, args.seed)
+    if args.end_to_end:
+        log("end to end: encode + file write")
+        res["end_to_end"] = end_to_end(ctx, torch, cfg, args.seed)
     if args.serve_requests > 0:
         log(f"serving loop: {args.serve_requests} requests at {args.serve_rate}/s")
         res["serving"] = serve_bench(ctx, torch, args.serve_requests, args.serve_rate, args.seed)
@@ -466,7 +562,7 @@ def main():
         res["batched"] = batched_bench(ctx, torch, args.seed)
     if not args.no_cpu_baseline:
         log("CPU baseline")
-        res["cpu_baseline"] = cpu_baseline(ctx, torch, args.config, args.seed)
+        res["cpu_baseline"] = cpu_baseline(ctx, torch, args.seed)
     print(json.dumps(res))
 
 

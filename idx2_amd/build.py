@@ -23,26 +23,34 @@ FLAGS = [
 ]
 
 
-def build(force=True, verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    if not force and os.path.exists(LIB):
-        t = os.path.getmtime(LIB)
+# Profiling variant with the encode kernel's phase clocks (-DIDX2G_PHASE_TIMERS,
+# idx2g_phase_times); never loaded by the product.
+PHASE_DIR = os.path.join(HERE, "_build_phase")
+PHASE_LIB = os.path.join(PHASE_DIR, "libidx2g.so")
+
+
+def build(force=True, verbose=False, phase_timers=False):
+    out_dir, lib = (PHASE_DIR, PHASE_LIB) if phase_timers else (OUT_DIR, LIB)
+    flags = FLAGS + (["-DIDX2G_PHASE_TIMERS"] if phase_timers else [])
+    os.makedirs(out_dir, exist_ok=True)
+    if not force and os.path.exists(lib):
+        t = os.path.getmtime(lib)
         if all(os.path.getmtime(d) <= t for d in DEPS):
-            return LIB
+            return lib
     objs = []
     for src in SRC:
-        obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-        cmd = [HIPCC, *FLAGS, "-c", "-x", "hip" if src.endswith(".hip") else "c++", src, "-o", obj]
+        obj = os.path.join(out_dir, os.path.basename(src) + ".o")
+        cmd = [HIPCC, *flags, "-c", "-x", "hip" if src.endswith(".hip") else "c++", src, "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], "-c", src, "-o", obj]
+            cmd = [HIPCC, *[f for f in flags if f != "-shared"], "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         subprocess.run(cmd, check=True)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp, "-ldl"], check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -506,10 +506,13 @@ constexpr int kParkN = 12;
 __device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& p, double* park) {
   xpass<kExt>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(1);
   ypass<kExt>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(2);
   zpass<kExt>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(3);
   sub_copy<true>(c, S, p);
 #pragma unroll
   for (int k = 0; k < kParkN; ++k) park[k * kTh + p.tid] = c[kN - kParkN + k];
@@ -522,22 +525,29 @@ __device__ __forceinline__ void extrapolate(double (&c)[kN], Lds& S, const Pos& 
   for (int k = 0; k < kParkN; ++k) c[kN - kParkN + k] = park[k * kTh + p.tid];
   sub_copy<false>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(4);
   zpass<kInv>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(5);
   ypass<kInv>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(6);
   xpass<kInv>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(7);
 }
 
 // One-level forward transform of the extended brick (X, Y, Z), no weights.
 __device__ __forceinline__ void forward(double (&c)[kN], Lds& S, const Pos& p) {
   xpass<kFwd>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(8);
   ypass<kFwd>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(9);
   zpass<kFwd>(c, S, p);
   __syncthreads();
+  IDX2G_PHASE(10);
 }
 
 // Inverse one-level transform (Z, Y, X), weights already removed.

@@ -564,8 +564,10 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, int seq, 
   if (p.tid == 0) { C.klo = 0x7fffffff; C.khi = -0x7fffffff; }
   if (p.tid < kMaxKeys) { C.kbits[p.tid] = 0; C.kfirst[p.tid] = 0x7fffffff; }
   __syncthreads();
+  IDX2G_PHASE(12);
   block_stage<DT>(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
   __syncthreads();
+  IDX2G_PHASE(13);
   // ---- 4. per-key offsets over the blocks (Morton order) ----
   const int klo = C.klo, khi = C.khi;
   const int nk = khi >= klo ? khi - klo + 1 : 0;
@@ -683,6 +685,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, int seq, 
     }
   }
   __syncthreads();
+  IDX2G_PHASE(14);
   if (C.over_s[slot]) {  // no room in the arena (reported through the counters): leave the buffer zero
     if (staged)
       for (uint32_t i = p.tid; i < nwords; i += kTh) words[i] = 0;

@@ -11,6 +11,34 @@
 
 namespace idx2g {
 
+// Phase clocks of the on-chip encode kernel (profiling builds only, compiled
+// with -DIDX2G_PHASE_TIMERS; see idx2g_phase_times): thread 0 of each
+// workgroup adds the wall-clock ticks (100 MHz) since its previous mark, taken
+// right after a workgroup barrier, to the phase's counter.
+#ifdef IDX2G_PHASE_TIMERS
+__device__ unsigned long long g_phase_ticks[32];
+__shared__ long long g_phase_prev;
+#define IDX2G_PHASE(n)                                                                          \
+  do {                                                                                          \
+    if (threadIdx.x == 0) {                                                                     \
+      const long long t_ = wall_clock64();                                                      \
+      atomicAdd(&::idx2g::g_phase_ticks[n], (unsigned long long)(t_ - ::idx2g::g_phase_prev)); \
+      ::idx2g::g_phase_prev = t_;                                                               \
+    }                                                                                           \
+  } while (0)
+#define IDX2G_PHASE_START()                                         \
+  do {                                                              \
+    if (threadIdx.x == 0) ::idx2g::g_phase_prev = wall_clock64();   \
+  } while (0)
+#else
+#define IDX2G_PHASE(n) \
+  do {                 \
+  } while (0)
+#define IDX2G_PHASE_START() \
+  do {                      \
+  } while (0)
+#endif
+
 typedef uint64_t u64;
 typedef int64_t i64;
 typedef uint32_t u32;

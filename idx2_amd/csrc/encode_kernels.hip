@@ -368,6 +368,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
   if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0, 0};
   const int total = start[njobs];
   int cj = -1;
+  IDX2G_PHASE_START();
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
     const int j = job_of(start, njobs, g);
     const EncArgs& a = table[j];
@@ -416,6 +417,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
       store_minmax(a, sh, dense, mn, mx);
     }
     __syncthreads();
+    IDX2G_PHASE(0);
     // LDS past the subgrid is free until the coding stage (it belongs to CodeLds)
     static_assert(sizeof(S.u.sub) + 8 + sizeof(double) * b32::kParkN * b32::kTh <= sizeof(b32::CodeLds),
                   "park area past the subgrid");
@@ -429,9 +431,11 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
       for (int z = 0; z < 16; ++z) dst[po + (unsigned)z * pplane] = c[2 * z];
     }
     b32::weigh(c, S, p, w8);
+    IDX2G_PHASE(11);
     b32::code_brick<DT>(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
                     reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride), s_blocks, cur);
     __syncthreads();
+    IDX2G_PHASE(15);
   }
   b32::flush_cursor(cur, table[0].counters);
 }

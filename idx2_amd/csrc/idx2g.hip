@@ -121,3 +121,14 @@ static int noalloc_setup(int device);  // noalloc.inc
 #include "server.inc"
 #include "tune.inc"
 #include "noalloc.inc"
+
+#ifdef IDX2G_PHASE_TIMERS
+// Profiling builds: the accumulated phase ticks of k_encode_b32 (10 ns each,
+// summed over workgroups) since the previous call, which clears them.
+extern "C" int idx2g_phase_times(unsigned long long* ticks32) {
+  HIPCHK(hipMemcpyFromSymbol(ticks32, HIP_SYMBOL(idx2g::g_phase_ticks), 32 * sizeof(unsigned long long)));
+  unsigned long long z[32] = {0};
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(idx2g::g_phase_ticks), z, sizeof(z)));
+  return IDX2G_OK;
+}
+#endif

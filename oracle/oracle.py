@@ -102,25 +102,50 @@ def ref_available():
     return os.path.exists(REF_APP)
 
 
-def ref_encode(workdir, name, field, data, brick=(32, 32, 32), nlevels=1, tolerance=1e-3,
-               bricks_per_chunk=4096, chunks_per_file=64, files_per_dir=64):
-    """Run the reference idx2App (built from the reference sources into oracle/_ref)."""
+def _pinned(cpus):
+    """preexec_fn restricting the child process to the CPUs in `cpus` (None: unchanged)."""
+    if cpus is None:
+        return None
+    return lambda: os.sched_setaffinity(0, cpus)
+
+
+def ref_write_raw(workdir, name, field, data):
+    """The raw input file the reference idx2App reads, named as it expects."""
     data = np.ascontiguousarray(data)
     nz, ny, nx = data.shape
     tname = "Float64" if data.dtype == np.float64 else "Float32"
     raw = os.path.join(workdir, f"{name}-{field}-[{nx}-{ny}-{nz}]-{tname}.raw")
     os.makedirs(workdir, exist_ok=True)
     data.tofile(raw)
-    out_dir = os.path.join(workdir, "ref")
+    return raw
+
+
+def ref_encode_raw(raw, out_dir, brick=(32, 32, 32), nlevels=1, tolerance=1e-3, bricks_per_chunk=4096,
+                   chunks_per_file=64, files_per_dir=64, cpus=None):
+    """Run the reference idx2App (built from the reference sources into
+    oracle/_ref) on a raw file written by ref_write_raw."""
     cmd = [REF_APP, "--encode", raw, "--num_levels", str(nlevels), "--tolerance", repr(tolerance),
            "--brick_size", *[str(b) for b in brick], "--bricks_per_chunk", str(bricks_per_chunk),
            "--chunks_per_file", str(chunks_per_file), "--files_per_dir", str(files_per_dir),
            "--out_dir", out_dir]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    return r.returncode, out_dir
+    r = subprocess.run(cmd, capture_output=True, text=True, preexec_fn=_pinned(cpus))
+    return r.returncode
 
 
-def ref_decode(workdir, idx2_path, first=None, last=None, downsampling=(0, 0, 0), tolerance=0.0):
+def ref_encode(workdir, name, field, data, brick=(32, 32, 32), nlevels=1, tolerance=1e-3,
+               bricks_per_chunk=4096, chunks_per_file=64, files_per_dir=64):
+    """Write the raw file and run the reference idx2App's encode on it."""
+    raw = ref_write_raw(workdir, name, field, data)
+    out_dir = os.path.join(workdir, "ref")
+    rc = ref_encode_raw(raw, out_dir, brick, nlevels, tolerance, bricks_per_chunk, chunks_per_file, files_per_dir)
+    return rc, out_dir
+
+
+def ref_decode(workdir, idx2_path, first=None, last=None, downsampling=(0, 0, 0), tolerance=0.0, parallel=False,
+               cpus=None):
+    """The reference idx2App's decode (serial, or its thread-pool decoder
+    with parallel=True) into workdir/refdec/out.raw; cpus: CPU set the
+    process is restricted to."""
     out_dir = os.path.join(workdir, "refdec")
     shutil.rmtree(out_dir, ignore_errors=True)
     os.makedirs(out_dir)
@@ -128,7 +153,9 @@ def ref_decode(workdir, idx2_path, first=None, last=None, downsampling=(0, 0, 0)
            "--downsampling", *[str(d) for d in downsampling], "--tolerance", repr(tolerance)]
     if first is not None:
         cmd += ["--first", *[str(v) for v in first], "--last", *[str(v) for v in last]]
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    if parallel:
+        cmd += ["--parallel"]
+    r = subprocess.run(cmd, capture_output=True, text=True, preexec_fn=_pinned(cpus))
     if r.returncode != 0:
         return r.returncode, None
     return 0, os.path.join(out_dir, "out.raw")
