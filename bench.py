@@ -122,8 +122,83 @@ def profile_counter(kernel, key):
     return None
 
 
-def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=48 << 30):
+def _serve_run(torch, sets, reqs, rate, slots, budget, max_out):
+    """Poisson arrivals of reqs at `rate`; the client holds at most max_out
+    bytes of output buffers (an overloaded server makes it wait: stalls)."""
     import threading
+    from idx2_amd import api
+    server = api.Server(0, hbm_budget=budget, slots=slots)
+    lat, qms, out_bytes = [], [], 0
+    pending = []
+    lock = threading.Condition()
+    done_flag = [False]
+
+    def waiter():
+        nonlocal out_bytes
+        while True:
+            with lock:
+                while not pending and not done_flag[0]:
+                    lock.wait()
+                if not pending:
+                    return
+                rid, buf, nb = pending.pop(0)
+            l_, q_, _ = server.wait(rid)
+            lat.append(l_)
+            qms.append(q_)
+            out_bytes += nb  # this request's output grid
+            del buf
+
+    th = threading.Thread(target=waiter, daemon=True)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    stalls = 0
+    try:
+        for k, q, gap in reqs:
+            ds, _, dt = sets[k]
+            t_next += gap * 400.0 / rate  # the stream's gaps are drawn at 400/s
+            now = time.perf_counter()
+            if t_next > now:
+                time.sleep(t_next - now)
+            dims, nbytes = ds.output_dims(**q)
+            while True:
+                with lock:
+                    held = sum(b.numel() * b.element_size() for _, b, _ in pending)
+                if held + nbytes <= max_out or held == 0:
+                    break
+                stalls += 1
+                time.sleep(0.001)
+            buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
+                              dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+            rid = server.submit(ds, buf.data_ptr(), out_owner=buf, **q)
+            with lock:
+                pending.append((rid, buf, nbytes))
+                lock.notify()
+    finally:
+        with lock:
+            done_flag[0] = True
+            lock.notify()
+        th.join()
+    wall = time.perf_counter() - t0
+    info = server.info()
+    cnt = server.counters()
+    server.close()
+    lat = np.array(lat)
+    return {"requests": len(reqs), "rate_per_s": rate,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_queue_ms": round(float(np.mean(qms)), 3), "served_per_s": round(len(reqs) / wall, 2),
+            "output_gbps": round(out_bytes / wall / 1e9, 3),
+            "peak_concurrent_batches": info["peak_running"], "peak_admitted_bytes": info["peak_bytes"],
+            "batches": cnt["batches"], "mean_batch": round(cnt["batched_requests"] / max(1, cnt["batches"]), 2),
+            "bypasses": cnt["bypasses"], "cache_hits": cnt["cache_hits"], "cache_misses": cnt["cache_misses"],
+            "cache_bytes": cnt["cache_bytes"], "client_stalls": stalls}
+
+
+def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=48 << 30,
+                sweep=(800.0, 1600.0, 3200.0, 6400.0), sweep_requests=2000):
+    """The serving loop on its three resident datasets: n_requests of the
+    seeded request stream at `rate`, then a sweep of higher rates
+    (sweep_requests each) up to saturation."""
     from idx2_amd import api, workloads
     sets = []
     for i, (dims, dt) in enumerate(workloads.SERVE_SETS):
@@ -135,74 +210,19 @@ def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, m
         del buf
         sets.append((ds, dims, dt))
     torch.cuda.synchronize()
-    reqs = [(sets[k][0], sets[k][2], q, gap) for k, q, gap in workloads.serve_requests(seed, n_requests, rate)]
-    server = api.Server(0, hbm_budget=budget, slots=slots)
-    lat, qms, out_bytes, read_bytes = [], [], 0, 0
-    pending = []
-    lock = threading.Condition()
-    done_flag = [False]
-
-    def waiter():
-        nonlocal out_bytes, read_bytes
-        while True:
-            with lock:
-                while not pending and not done_flag[0]:
-                    lock.wait()
-                if not pending:
-                    return
-                rid, buf = pending.pop(0)
-            l_, q_, st = server.wait(rid)
-            lat.append(l_)
-            qms.append(q_)
-            out_bytes += st["raw_bytes"]
-            read_bytes += st["chunk_bytes"]
-            del buf
-
-    th = threading.Thread(target=waiter, daemon=True)
-    th.start()
-    t0 = time.perf_counter()
-    t_next = t0
-    stalls = 0
-    try:
-        for ds, dt, q, gap in reqs:
-            t_next += gap
-            now = time.perf_counter()
-            if t_next > now:
-                time.sleep(t_next - now)
-            dims, nbytes = ds.output_dims(**q)
-            # the client holds at most max_out bytes of output buffers (an
-            # overloaded server makes it wait here; counted as stalls)
-            while True:
-                with lock:
-                    held = sum(b.numel() * b.element_size() for _, b in pending)
-                if held + nbytes <= max_out or held == 0:
-                    break
-                stalls += 1
-                time.sleep(0.001)
-            buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
-                              dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
-            rid = server.submit(ds, buf.data_ptr(), **q)
-            with lock:
-                pending.append((rid, buf))
-                lock.notify()
-    finally:
-        with lock:
-            done_flag[0] = True
-            lock.notify()
-        th.join()
-    wall = time.perf_counter() - t0
-    info = server.info()
-    server.close()
+    reqs = workloads.serve_requests(seed, n_requests, rate)
+    res = _serve_run(torch, sets, reqs, rate, slots, budget, max_out)
+    res.update({"slots": slots, "hbm_budget_bytes": budget,
+                "datasets": [list(d) + [t] for d, t in workloads.SERVE_SETS]})
+    if sweep:
+        res["sweep"] = []
+        for r in sweep:
+            log(f"serving sweep: {sweep_requests} requests at {r}/s")
+            res["sweep"].append(_serve_run(torch, sets, workloads.serve_requests(seed + 1, sweep_requests, 400.0), r,
+                                           slots, budget, max_out))
     for ds, _, _ in sets:
         ds.free()
-    lat = np.array(lat)
-    return {"requests": n_requests, "rate_per_s": rate, "slots": slots, "hbm_budget_bytes": budget,
-            "datasets": [list(d) + [t] for d, t in workloads.SERVE_SETS],
-            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "mean_queue_ms": round(float(np.mean(qms)), 3), "served_per_s": round(n_requests / wall, 2),
-            "output_gbps": round(out_bytes / wall / 1e9, 3), "stream_bytes_read": int(read_bytes),
-            "peak_concurrent": info["peak_running"], "peak_admitted_bytes": info["peak_bytes"],
-            "client_stalls": stalls}
+    return res
 
 
 def batched_fields(torch, ctx, seed, n_fields=64):

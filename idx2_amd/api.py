@@ -57,6 +57,13 @@ class Stats(ctypes.Structure):
                 ("chunk_bytes", ctypes.c_int64), ("exp_bytes", ctypes.c_int64)]
 
 
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_int64), ("batches", ctypes.c_int64), ("batched_requests", ctypes.c_int64),
+                ("bypasses", ctypes.c_int64), ("cache_hits", ctypes.c_int64), ("cache_misses", ctypes.c_int64),
+                ("cache_inserts", ctypes.c_int64), ("cache_evictions", ctypes.c_int64),
+                ("cache_bytes", ctypes.c_int64), ("peak_bytes", ctypes.c_int64), ("peak_running", ctypes.c_int32)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("enc_wg_per_cu", ctypes.c_int32), ("dec_wg_per_cu", ctypes.c_int32), ("walk_batch", ctypes.c_int32)]
 
@@ -67,7 +74,7 @@ EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_e
            "idx2g_synth_field", "idx2g_server_create", "idx2g_server_submit", "idx2g_server_wait",
            "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint", "idx2g_encode_batch",
            "idx2g_decode_batch", "idx2g_validate_params", "idx2g_autotune", "idx2g_set_tuning",
-           "idx2g_get_tuning", "idx2g_decode_noalloc"]
+           "idx2g_get_tuning", "idx2g_decode_noalloc", "idx2g_server_counters"]
 
 _lib = None
 
@@ -104,6 +111,7 @@ def lib():
         L.idx2g_output_dims.argtypes = [vp, ctypes.POINTER(Query), ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.idx2g_decode.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.c_int]
         L.idx2g_decode_noalloc.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.POINTER(i64)]
+        L.idx2g_server_counters.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.idx2g_last_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.idx2g_dataset_chunk_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(i64)]
         L.idx2g_synth_field.argtypes = [vp, vp, ctypes.POINTER(i32), i32, ctypes.POINTER(ctypes.c_double),
@@ -388,6 +396,12 @@ class Server:
         finally:
             self._held.pop(int(rid), None)
         return lat.value, qms.value, {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def counters(self):
+        """Batches, bypasses and coarse-level cache activity of the serving loop."""
+        c = ServerStats()
+        _check(lib().idx2g_server_counters(self._h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in ServerStats._fields_}
 
     def info(self):
         pb, pr, done = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int64()

@@ -2,6 +2,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -143,6 +144,11 @@ bool arena_free(ScratchArena& A, void* p) {
 
 void set_thread_arena(ScratchArena* a) { tl_arena = a; }
 
+uint64_t next_dataset_uid() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1);
+}
+
 hipError_t dev_alloc(void** p, size_t bytes) {
   if (tl_arena) return arena_alloc(*tl_arena, p, bytes);
   DevCache& c = dev_cache();
@@ -183,6 +189,59 @@ void dev_free(void* p) {
   auto it = c.live.find(p);
   if (it == c.live.end()) { (void)hipFree(p); return; }
   c.free_[it->second.first].emplace(it->second.second, p);
+  c.live.erase(it);
+}
+
+// Pinned host blocks kept for reuse (file writing stages whole chunk blobs
+// through them; pinning hundreds of MB per call would cost more than the
+// copy).  Blocks are returned by pinned_free and reused by the next request
+// of at most 1.5x their size.
+namespace {
+struct PinCache {
+  std::mutex mu;
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> live;
+};
+PinCache& pin_cache() {
+  static PinCache* c = new PinCache();
+  return *c;
+}
+}  // namespace
+
+hipError_t pinned_alloc(void** p, size_t bytes) {
+  PinCache& c = pin_cache();
+  const size_t n = round_bytes(bytes ? bytes : 1);
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.free_.lower_bound(n);
+    if (it != c.free_.end() && it->first <= n + n / 2) {
+      *p = it->second;
+      c.live[*p] = it->first;
+      c.free_.erase(it);
+      return hipSuccess;
+    }
+  }
+  hipError_t e = hipHostMalloc(p, n, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> g(c.mu);  // give the cached blocks back and retry
+    for (auto& kv : c.free_) (void)hipHostFree(kv.second);
+    c.free_.clear();
+    e = hipHostMalloc(p, n, hipHostMallocDefault);
+  }
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(c.mu);
+  c.live[*p] = n;
+  return hipSuccess;
+}
+
+void pinned_free(void* p) {
+  if (!p) return;
+  PinCache& c = pin_cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  auto it = c.live.find(p);
+  if (it == c.live.end()) { (void)hipHostFree(p); return; }
+  c.free_.emplace(it->second, p);
   c.live.erase(it);
 }
 

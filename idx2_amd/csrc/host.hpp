@@ -56,8 +56,13 @@ struct idx2g_ctx {
   bool noalloc = false;
 };
 
+namespace idx2g {
+uint64_t next_dataset_uid();  // process-unique id (cache keys never confuse two datasets)
+}
+
 struct idx2g_dataset {
   idx2g::Layout L;
+  uint64_t uid = idx2g::next_dataset_uid();
   int device = 0;
   idx2g::LevelData lv[idx2g::kMaxLevels];
   uint32_t* d_blocks = nullptr;  // packed block tables
@@ -105,6 +110,9 @@ void set_thread_arena(ScratchArena* a);
 // library call that freed them has synchronised its stream.
 hipError_t dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
+// Pinned host memory through a process-wide cache of freed blocks.
+hipError_t pinned_alloc(void** p, size_t bytes);
+void pinned_free(void* p);
 #define HIPCHK(x)                                                  \
   do {                                                             \
     hipError_t e_ = (x);                                           \

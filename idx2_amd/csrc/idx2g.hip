@@ -8,9 +8,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <fcntl.h>
+#include <list>
+#include <mutex>
 #include <map>
 #include <string>
 #include <sys/stat.h>
+#include <thread>
+#include <unistd.h>
 #include <tuple>
 #include <vector>
 

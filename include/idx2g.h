@@ -233,10 +233,17 @@ int idx2g_get_tuning(int hip_device, idx2g_tuning* t);
 /* ---- decode serving loop ----
  * No counterpart in the reference (its decoder serves one query per call,
  * Source/Core/idx2Decode.cpp:326-492, Source/idx2.h:43-44): a queue of decode
- * requests on resident datasets, admitted in arrival order as soon as one of
- * `slots` execution slots (one HIP stream each) is free and the request's
- * HBM footprint fits `hbm_budget_bytes` (a request that exceeds the budget
- * alone runs when nothing else holds memory).  Requests run concurrently;
+ * requests on resident datasets.  Each of `slots` execution slots (one HIP
+ * stream each) takes the requests that fit the HBM budget when it is free --
+ * up to 16, decoded together in shared launches (idx2g_decode_batch) --
+ * in arrival order, except that while the oldest request does not fit, up
+ * to 64 later ones that fit may pass it, at most 32 times in all (a request
+ * larger than the budget alone runs when nothing else holds memory).
+ * Coarse levels whose whole grid is at most 512 MB are decoded whole and
+ * kept in HBM (a quarter of the budget at most, least recently used first
+ * out), keyed by dataset, level, tolerance exponent and decode masks, so
+ * later requests with the same key start below them.  Requests run
+ * concurrently;
  * each result is the same as idx2g_decode's for the same query.  Datasets
  * are only read; idx2g_dataset_free on a dataset with requests queued or
  * running returns at once and the last of those requests releases it. */
@@ -252,6 +259,17 @@ int idx2g_server_wait(idx2g_server* s, int64_t request_id, double* latency_ms, d
                       idx2g_stats* stats);
 /* Peak admitted bytes, peak concurrent requests, completed requests. */
 int idx2g_server_info(idx2g_server* s, int64_t* peak_bytes, int* peak_running, int64_t* completed);
+typedef struct {
+  int64_t completed;
+  int64_t batches;           /* batched decodes run */
+  int64_t batched_requests;  /* requests in them (batched_requests / batches: mean batch) */
+  int64_t bypasses;          /* requests admitted ahead of a blocked older one */
+  int64_t cache_hits, cache_misses, cache_inserts, cache_evictions;
+  int64_t cache_bytes;       /* HBM held by cached coarse levels now */
+  int64_t peak_bytes;        /* peak admitted + cached bytes */
+  int32_t peak_running;      /* peak concurrent batches */
+} idx2g_server_stats;
+int idx2g_server_counters(idx2g_server* s, idx2g_server_stats* out);
 /* Drains the queue, then frees the server. */
 int idx2g_server_destroy(idx2g_server* s);
 /* The HBM footprint the server charges a request (output grid + working

@@ -1,6 +1,7 @@
 """Decode serving loop (idx2g_server) on the GPU: requests of different ROI /
 downsampling / tolerance on resident datasets run concurrently under an HBM
-budget, and every result is bit-identical to the same query decoded alone."""
+budget, batched into shared launches, with decoded coarse levels cached;
+every result is bit-identical to the same query decoded alone."""
 import numpy as np
 import pytest
 
@@ -68,13 +69,71 @@ def test_mix_equals_alone(resident):
     server = api.Server(0, hbm_budget=4 << 30, slots=4)
     got, lats = _run_mix(server, jobs)
     info = server.info()
+    cnt = server.counters()
     server.close()
     for (ds, q, want), g in zip(jobs, got):
         assert np.array_equal(g.view(np.uint8), want.reshape(-1).view(np.uint8)), q
     assert info["completed"] == len(jobs)
-    assert info["peak_running"] >= 2          # requests overlapped on the device
     assert info["peak_bytes"] <= 4 << 30
+    assert cnt["batched_requests"] == len(jobs)
+    assert cnt["batches"] < len(jobs)        # requests shared launches
     assert all(x > 0 for x in lats)
+
+
+def test_repeated_key_hits_the_level_cache(resident):
+    """A coarse level decoded whole for one request is reused by later
+    requests with the same tolerance and downsampling; their results equal
+    the same queries decoded alone (without the cache)."""
+    from idx2_amd import api
+    ctx, sets = resident
+    ds, dims = sets[0]  # 2 levels
+    qs = [dict(first=(5, 9, 3), last=(100, 77, 50), tolerance=1e-4),
+          dict(first=(0, 0, 0), last=(40, 40, 40), tolerance=1e-4),
+          dict(first=(60, 30, 20), last=(159, 127, 95), tolerance=1e-4)]
+    jobs = [(ds, q, ds.decode(**q)) for q in qs]
+    server = api.Server(0, hbm_budget=4 << 30, slots=1)
+    got0, _ = _run_mix(server, jobs[:1])   # decodes level 1 whole and keeps it
+    got1, _ = _run_mix(server, jobs[1:])
+    cnt = server.counters()
+    server.close()
+    for (_, q, want), g in zip(jobs, got0 + got1):
+        assert np.array_equal(g.view(np.uint8), want.reshape(-1).view(np.uint8)), q
+    assert cnt["cache_inserts"] >= 1
+    assert cnt["cache_hits"] >= 2
+    assert cnt["cache_bytes"] > 0
+
+
+def test_small_requests_pass_a_blocked_head(resident):
+    """While the oldest request does not fit next to the one admitted first,
+    a later small one that fits is admitted ahead of it; all results are
+    unchanged."""
+    import torch
+    from idx2_amd import api
+    ctx, sets = resident
+    ds, dims = sets[0]
+    big = dict(tolerance=1e-6)
+    small = dict(first=(0, 0, 0), last=(15, 15, 15), tolerance=1e-2)
+    fb, fs = api.decode_footprint(ds, **big), api.decode_footprint(ds, **small)
+    assert fs < fb
+    budget = fb + fs  # the first big request and a small one fit, two big ones do not
+    jobs = [(ds, big, ds.decode(**big)), (ds, big, ds.decode(**big)), (ds, small, ds.decode(**small))]
+    # two slots: while the first request runs (milliseconds), the second
+    # slot finds the second big one blocked at the head and the small one
+    # fitting behind it (the three submissions take microseconds)
+    server = api.Server(0, hbm_budget=budget, slots=2)
+    outs, ids = [], []
+    for d, q, want in jobs:
+        buf = torch.empty(want.size, dtype=torch.float32, device="cuda")
+        outs.append(buf)
+        ids.append(server.submit(d, buf.data_ptr(), out_owner=buf, **q))
+    for i in ids:
+        server.wait(i)
+    cnt = server.counters()
+    server.close()
+    torch.cuda.synchronize()
+    for (_, q, want), g in zip(jobs, outs):
+        assert np.array_equal(g.cpu().numpy().view(np.uint8), want.reshape(-1).view(np.uint8)), q
+    assert cnt["bypasses"] >= 1
 
 
 def test_budget_serialises(resident):
