@@ -14,6 +14,7 @@ import hashlib
 import json
 import os
 import shutil
+import subprocess
 import sys
 import tempfile
 
@@ -92,11 +93,52 @@ def make_case(name, spec):
         json.dump(meta, f, indent=1)
 
 
+# A dataset with one level-0 file deleted: the reference opens files as a
+# decode reaches them and fails with FileNotFound (idx2Read.cpp:252-270)
+# exactly when the query needs a brick of the missing file; the other
+# queries decode normally.
+MISSING = dict(source="c3d_f32_2lvl", removed="L00/B1/B2.bin",
+               queries=[dict(), dict(downsampling=(1, 1, 1)), dict(first=(0, 0, 0), last=(31, 31, 31)),
+                        dict(first=(40, 40, 40), last=(71, 67, 65)), dict(first=(0, 0, 0), last=(63, 63, 31)),
+                        dict(tolerance=1e-1), dict(first=(5, 0, 0), last=(60, 30, 20), downsampling=(1, 0, 1))])
+
+
+def make_missing_file_case():
+    src = os.path.join(HERE, MISSING["source"], "G")
+    with tempfile.TemporaryDirectory() as wd:
+        shutil.copytree(src, os.path.join(wd, "G"))
+        os.remove(os.path.join(wd, "G", "f", MISSING["removed"]))
+        idx = os.path.join(wd, "G", "f.idx2")
+        qs = []
+        for q in MISSING["queries"]:
+            out_dir = os.path.join(wd, "refdec")
+            shutil.rmtree(out_dir, ignore_errors=True)
+            os.makedirs(out_dir)
+            cmd = [O.REF_APP, "--decode", idx, "--out_dir", out_dir, "--out_file", "out.raw",
+                   "--downsampling", *[str(d) for d in q.get("downsampling", (0, 0, 0))],
+                   "--tolerance", repr(q.get("tolerance", 0.0))]
+            if "first" in q:
+                cmd += ["--first", *[str(v) for v in q["first"]], "--last", *[str(v) for v in q["last"]]]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            rec = {k: list(v) if isinstance(v, tuple) else v for k, v in q.items()}
+            if r.returncode == 0:
+                dec = np.fromfile(os.path.join(out_dir, "out.raw"), dtype=np.float32)
+                rec.update(result="ok", sha256=sha(dec), size=int(dec.size))
+            else:
+                msg = r.stdout + r.stderr
+                rec.update(result="FileNotFound" if "FileNotFound" in msg else "error")
+            qs.append(rec)
+    with open(os.path.join(HERE, "missing_file.json"), "w") as f:
+        json.dump(dict(source=MISSING["source"], removed=MISSING["removed"], queries=qs), f, indent=1)
+
+
 def main():
     assert O.ref_available(), "build the reference first: make -f oracle/ref.mk"
     for name, spec in CASES.items():
         make_case(name, spec)
         print("wrote", name)
+    make_missing_file_case()
+    print("wrote missing_file.json")
 
 
 if __name__ == "__main__":

@@ -200,14 +200,30 @@ def test_invalid_parameters_return_reference_codes(ctx):
         assert e.value.code == code, kw
 
 
-def test_missing_file_is_file_not_found(ctx):
+def test_missing_file_matches_reference(ctx):
+    """A dataset with one level-0 file deleted (tests/golden/missing_file.json,
+    written by the reference idx2App): it opens, queries that need a brick of
+    the missing file fail with FileNotFound as the reference's do, and the
+    others decode to the reference's bytes."""
+    import json
     from idx2_amd import api
+    fx = json.load(open(os.path.join(GOLDEN, "missing_file.json")))
     with tempfile.TemporaryDirectory() as wd:
-        shutil.copytree(os.path.join(GOLDEN, "c3d_f32_64", "G"), os.path.join(wd, "G"))
-        os.remove(os.path.join(wd, "G", "f", "L00.bin"))
-        with pytest.raises(api.Idx2Error) as e:
-            api.Dataset.open(ctx, os.path.join(wd, "G", "f.idx2"))
-        assert e.value.code == 37
+        shutil.copytree(os.path.join(GOLDEN, fx["source"], "G"), os.path.join(wd, "G"))
+        os.remove(os.path.join(wd, "G", "f", fx["removed"]))
+        ds = api.Dataset.open(ctx, os.path.join(wd, "G", "f.idx2"))
+        for q in fx["queries"]:
+            kw = {k: tuple(v) if isinstance(v, list) else v for k, v in q.items()
+                  if k in ("first", "last", "downsampling", "tolerance")}
+            print(f"  query {kw} -> {q['result']}", flush=True)
+            if q["result"] == "ok":
+                out = ds.decode(**kw)
+                assert out.size == q["size"] and _sha(out) == q["sha256"], q
+            else:
+                with pytest.raises(api.Idx2Error) as e:
+                    ds.decode(**kw)
+                assert e.value.code == 37, q  # FileNotFound
+        ds.free()
 
 
 def test_device_synth_matches_numpy(ctx):
