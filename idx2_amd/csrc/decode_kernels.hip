@@ -611,4 +611,43 @@ __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restric
   }
 }
 
+// Brick lists of the query boxes of one decode round, one thread per box
+// position (work item g of job j: box position g - start[j] in x-fastest
+// order).  A full brick goes to rank "full positions before it"; any other
+// to nfull + "other positions before it"; both counts are closed forms of
+// the position because the full bricks form the box's corner sub-box.
+__global__ void k_plan_bricks(const PlanArgs* __restrict__ jobs, const int* __restrict__ start, int njobs) {
+  const int total = start[njobs];
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+    const int j = job_of(start, njobs, g);
+    const PlanArgs& P = jobs[j];
+    const int p = g - start[j];
+    const int x = p % P.nbox[0], y = (p / P.nbox[0]) % P.nbox[1], z = p / (P.nbox[0] * P.nbox[1]);
+    const int fx = P.nfull3[0], fy = P.nfull3[1], fz = P.nfull3[2];
+    const int before = min(z, fz) * fx * fy + (z < fz ? min(y, fy) * fx + (y < fy ? min(x, fx) : 0) : 0);
+    const bool full = x < fx && y < fy && z < fz;
+    const int slot = full ? before : P.nfull + (p - before);
+    const int b3[3] = {P.bf[0] + x, P.bf[1] + y, P.bf[2] + z};
+    // linear id: the order's last position takes the lowest bit
+    int c[3] = {b3[0], b3[1], b3[2]};
+    u64 lin = 0;
+    for (int i = P.nbits - 1; i >= 0; --i) {
+      const int d = P.axis[i];
+      lin |= u64(c[d] & 1) << (P.nbits - 1 - i);
+      c[d] >>= 1;
+    }
+    int lo = 0, hi = P.nb_level - 1;  // rank among the sorted ids (every box brick exists)
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (P.sorted_lin[mid] < lin) lo = mid + 1;
+      else hi = mid;
+    }
+    P.xyz[3 * slot] = b3[0];
+    P.xyz[3 * slot + 1] = b3[1];
+    P.xyz[3 * slot + 2] = b3[2];
+    P.lin[slot] = lin;
+    P.dense[slot] = lo;
+  }
+}
+
 }  // namespace idx2g

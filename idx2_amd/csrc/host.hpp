@@ -28,6 +28,7 @@ struct LevelData {
   long long exp_off[8] = {0};
   uint64_t exp_bytes = 0;
   int* d_xyz = nullptr;
+  uint64_t* d_lin = nullptr;   // lin on the device (the query planner's rank lookups)
   // files of this level absent when the dataset was opened (sorted file ids):
   // a decode that needs one of their bricks fails with FileNotFound, as the
   // reference's does when it opens the file (idx2Read.cpp:269-270)

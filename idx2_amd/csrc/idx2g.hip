@@ -57,6 +57,7 @@ static void free_level(LevelData& lv) {
   if (lv.d_dir) dev_free(lv.d_dir);
   if (lv.d_exps) dev_free(lv.d_exps);
   if (lv.d_xyz) dev_free(lv.d_xyz);
+  if (lv.d_lin) dev_free(lv.d_lin);
   lv = LevelData();
 }
 
@@ -76,6 +77,16 @@ static void level_lists(idx2g_dataset* ds) {
     ds->L.level_bricks(l, &lv.lin, &lv.xyz);
     lv.nb = (int)lv.lin.size();
   }
+}
+
+// The levels' sorted brick ids on the device, for the decode's query planner.
+static int upload_level_ids(idx2g_dataset* ds) {
+  for (int l = 0; l < ds->L.nlevels; ++l) {
+    LevelData& lv = ds->lv[l];
+    TRY(dalloc(&lv.d_lin, lv.lin.size()));
+    if (!lv.lin.empty()) HIPCHK(hipMemcpy(lv.d_lin, lv.lin.data(), lv.lin.size() * 8, hipMemcpyHostToDevice));
+  }
+  return IDX2G_OK;
 }
 
 static void exp_layout(idx2g_dataset* ds) {
@@ -112,6 +123,21 @@ static bool full_2d32(const Layout& L, const int* b3, const int gd[3]) {
     for (int l = 0; l < L.nlevels; ++l)
       if ((int)L.blocks[l][s].size() * b2d::kGroup > kMaxBlocks) return false;
   return true;
+}
+
+// Per axis, the brick coordinate below which a brick of a level whose grid is
+// gd is full: the bricks full_b32 / full_2d32 accept are exactly those below
+// F on every axis.  All zero when the layout takes no on-chip path.
+static void full_extent(const Layout& L, const int gd[3], int F[3]) {
+  const int origin[3] = {0, 0, 0};
+  F[0] = F[1] = F[2] = 0;
+  if (full_b32(L, origin, gd)) {
+    for (int d = 0; d < 3; ++d) F[d] = gd[d] / 32;
+  } else if (full_2d32(L, origin, gd)) {
+    F[0] = gd[0] / 32;
+    F[1] = gd[1] / 32;
+    F[2] = 1;
+  }
 }
 
 }  // namespace idx2g

@@ -123,6 +123,24 @@ struct ChunkRec {
   int sb, key, chunk;
 };
 
+// Device-side query planning of one (query, level): the bricks of the query
+// box [bf, bf + nbox) in brick coordinates, the full bricks first (those in
+// the box's corner [bf, bf + nfull3), which the on-chip kernels take), then
+// the rest, each with its linear id (bit interleave of the level's brick
+// order, GetLinearBrick, Source/Core/idx2Lookup.cpp:13-40) and dense index
+// (rank among the level's sorted linear ids).
+struct PlanArgs {
+  int bf[3], nbox[3], nfull3[3];
+  int nfull, n;
+  int nbits;                 // length of the level's brick order
+  signed char axis[64];      // axis of each order position, most significant first
+  const u64* sorted_lin;     // the level's bricks, sorted linear ids
+  int nb_level;
+  int* xyz;                  // out [n][3]
+  u64* lin;                  // out [n]
+  int* dense;                // out [n]
+};
+
 struct DecArgs {
   BrickConst k;
   PermTables perm;
