@@ -74,7 +74,10 @@ EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_e
            "idx2g_synth_field", "idx2g_server_create", "idx2g_server_submit", "idx2g_server_wait",
            "idx2g_server_info", "idx2g_server_destroy", "idx2g_decode_footprint", "idx2g_encode_batch",
            "idx2g_decode_batch", "idx2g_validate_params", "idx2g_autotune", "idx2g_set_tuning",
-           "idx2g_get_tuning", "idx2g_decode_noalloc", "idx2g_server_counters"]
+           "idx2g_get_tuning", "idx2g_decode_noalloc", "idx2g_server_counters",
+           "idx2g_decode_bricks", "idx2g_brick_pool_count", "idx2g_brick_pool_key", "idx2g_brick_pool_find",
+           "idx2g_brick_pool_lookup", "idx2g_brick_pool_resolution", "idx2g_brick_pool_copy", "idx2g_brick_pool_write",
+           "idx2g_brick_pool_free"]
 
 _lib = None
 
@@ -112,6 +115,16 @@ def lib():
         L.idx2g_decode.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.c_int]
         L.idx2g_decode_noalloc.argtypes = [vp, vp, ctypes.POINTER(Query), vp, ctypes.POINTER(i64)]
         L.idx2g_server_counters.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        i3 = ctypes.c_int * 3
+        L.idx2g_decode_bricks.argtypes = [vp, vp, ctypes.POINTER(Query), ctypes.POINTER(vp)]
+        L.idx2g_brick_pool_count.argtypes = [vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.idx2g_brick_pool_key.argtypes = [vp, ctypes.c_int, i3, ctypes.POINTER(ctypes.c_uint64)]
+        L.idx2g_brick_pool_find.argtypes = [vp, ctypes.c_int, i3, ctypes.POINTER(vp), i3]
+        L.idx2g_brick_pool_lookup.argtypes = [vp, vp, i64, vp, vp]
+        L.idx2g_brick_pool_copy.argtypes = [vp, ctypes.c_int, i3, vp, ctypes.c_int]
+        L.idx2g_brick_pool_resolution.argtypes = [vp, i3, ctypes.POINTER(ctypes.c_int)]
+        L.idx2g_brick_pool_write.argtypes = [vp, ctypes.c_char_p]
+        L.idx2g_brick_pool_free.argtypes = [vp]
         L.idx2g_last_stats.argtypes = [vp, ctypes.POINTER(Stats)]
         L.idx2g_dataset_chunk_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(i64)]
         L.idx2g_synth_field.argtypes = [vp, vp, ctypes.POINTER(i32), i32, ctypes.POINTER(ctypes.c_double),
@@ -304,6 +317,14 @@ class Dataset:
         _check(lib().idx2g_decode(self.ctx._h, self._h, ctypes.byref(q), out.ctypes.data_as(ctypes.c_void_p), 0))
         return out
 
+    def decode_bricks(self, **query):
+        """Sparse-brick decode (the reference's HashMap output mode): a
+        BrickPool of the significant bricks of every decoded level."""
+        q = make_query(**query)
+        h = ctypes.c_void_p()
+        _check(lib().idx2g_decode_bricks(self.ctx._h, self._h, ctypes.byref(q), ctypes.byref(h)))
+        return BrickPool(h, self.info()["nlevels"])
+
     def chunk_bytes(self):
         n = ctypes.c_int64()
         _check(lib().idx2g_dataset_chunk_bytes(self._h, None, ctypes.byref(n)))
@@ -311,6 +332,63 @@ class Dataset:
         _check(lib().idx2g_dataset_chunk_bytes(self._h, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                                ctypes.byref(n)))
         return buf
+
+
+class BrickPool:
+    """Bricks kept by Dataset.decode_bricks (idx2g_brick_pool)."""
+
+    def __init__(self, h, nlevels):
+        self._h = h
+        self.nlevels = nlevels
+
+    def count(self, level):
+        """(kept bricks, bricks of the level's grid)"""
+        k, t = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().idx2g_brick_pool_count(self._h, level, ctypes.byref(k), ctypes.byref(t)))
+        return k.value, t.value
+
+    def key(self, level, brick3):
+        k = ctypes.c_uint64()
+        _check(lib().idx2g_brick_pool_key(self._h, level, (ctypes.c_int * 3)(*brick3), ctypes.byref(k)))
+        return k.value
+
+    def find(self, level, brick3):
+        """(device address, dims (x, y, z)) of a kept brick, None when not kept."""
+        ptr, dims = ctypes.c_void_p(), (ctypes.c_int * 3)()
+        rc = lib().idx2g_brick_pool_find(self._h, level, (ctypes.c_int * 3)(*brick3), ctypes.byref(ptr), dims)
+        if rc == 7:  # AttributeNotFound
+            return None
+        _check(rc)
+        return ptr.value, tuple(dims)
+
+    def samples(self, level, brick3):
+        """A kept brick's samples as a numpy float64 array (z, y, x); None when not kept."""
+        f = self.find(level, brick3)
+        if f is None:
+            return None
+        dims = f[1]
+        out = np.empty((dims[2], dims[1], dims[0]), dtype=np.float64)
+        _check(lib().idx2g_brick_pool_copy(self._h, level, (ctypes.c_int * 3)(*brick3),
+                                           out.ctypes.data_as(ctypes.c_void_p), 0))
+        return out
+
+    def lookup(self, ctx, n, keys_dev_ptr, out_dev_ptr):
+        """Device-side lookup of n brick keys (u64 device array) into device addresses (u64 device array)."""
+        _check(lib().idx2g_brick_pool_lookup(ctx._h, self._h, n, ctypes.c_void_p(keys_dev_ptr),
+                                             ctypes.c_void_p(out_dev_ptr)))
+
+    def resolution(self, brick3):
+        r = ctypes.c_int()
+        _check(lib().idx2g_brick_pool_resolution(self._h, (ctypes.c_int * 3)(*brick3), ctypes.byref(r)))
+        return r.value
+
+    def write(self, path):
+        _check(lib().idx2g_brick_pool_write(self._h, path.encode()))
+
+    def free(self):
+        if self._h:
+            _check(lib().idx2g_brick_pool_free(self._h))
+            self._h = None
 
 
 def autotune(ctx):

@@ -11,7 +11,7 @@ DEPS = SRC + [os.path.join(HERE, "csrc", f) for f in (
     "device.hpp", "kernels.hpp", "brick32.hpp", "code32.hpp", "dec32.hpp", "brick2d.hpp", "code2d.hpp",
     "dec2d.hpp", "host.hpp", "layout.hpp", "encode_kernels.hip",
     "decode_kernels.hip", "encode.inc", "files.inc", "decode.inc", "api.inc", "server.inc", "tune.inc",
-    "noalloc.inc")] + [os.path.join(ROOT, "include", "idx2g.h")]
+    "noalloc.inc", "pool.inc")] + [os.path.join(ROOT, "include", "idx2g.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

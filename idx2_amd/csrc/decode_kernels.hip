@@ -83,7 +83,7 @@ __device__ inline void dunpack(uint32_t w, int org[3], int dims[3], int* skip, i
 __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __restrict__ table,
                                                            const int* __restrict__ start, int njobs) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
-  __shared__ int s_klo, s_khi;
+  __shared__ int s_klo, s_khi, s_sig;
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int prec3[4] = {63, 62, 61, 60};
   const int total = start[njobs];
@@ -102,6 +102,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
     const u64 lin = a.lin[bi];
     const int dense = a.dense[bi];
     for (int i = tid; i < k.nvox; i += blockDim.x) F[i] = 0;
+    if (tid == 0) s_sig = 0;
     __syncthreads();
     for (int sb = 0; sb < k.nsb; ++sb) {
       if (!((a.mask >> sb) & 1)) continue;
@@ -180,6 +181,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
                   const u64 X = decode_plane(a.blob, pos, n, nv);
                   n_out = n;
                   S.pw[b * npl + (63 - bp)] = X;
+                  S.nbps[b] = 1;  // a plane of the block was decoded
                 }
               } else if (bp > 63) {
                 n_out = 0;
@@ -204,6 +206,9 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
         const int stop = max(64 - k.endbp, 57 + a.etol - e);
         const int np = stop > 63 ? 0 : 64 - stop;
         if (np == 0 || bypass) continue;
+        // a significant block (DecodeSubband, idx2Decode.cpp:205-212): planes
+        // decoded, on a detail subband or on the coarsest level
+        if (lane == 0 && S.nbps[b] && (sb > 0 || a.coarsest)) s_sig = 1;
         const int nd = (bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1);
         const int nv = 1 << (2 * nd);
         u64 u = 0;
@@ -224,6 +229,11 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
       __syncthreads();
     }
     inverse_brick(k, F, a.winv, a.coarsest);
+    if (a.pool_vol) {  // the whole extended brick into its pool slot
+      double* dst = a.pool_vol + (size_t)bi * k.nvox;
+      for (int i = tid; i < k.nvox; i += blockDim.x) dst[i] = F[i];
+      if (tid == 0) a.pool_sig[bi] = (uint8_t)s_sig;
+    }
     // data region into this level's grid (read by the next finer level)
     if (a.cgrid) {
       int ext[3];
@@ -647,6 +657,63 @@ __global__ void k_plan_bricks(const PlanArgs* __restrict__ jobs, const int* __re
     P.xyz[3 * slot + 2] = b3[2];
     P.lin[slot] = lin;
     P.dense[slot] = lo;
+  }
+}
+
+// Sparse-brick pool: brick b of dst is slot slots[b] of src (nvox doubles each).
+__global__ void k_gather_bricks(const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ slots,
+                                int nb, int nvox) {
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const double* s = src + (size_t)slots[b] * nvox;
+    double* d = dst + (size_t)b * nvox;
+    for (int i = threadIdx.x; i < nvox; i += blockDim.x) d[i] = s[i];
+  }
+}
+
+// The pool's device hash table: open addressing with linear probing over a
+// power-of-two table at most half full; a slot holds key + 1 (0: empty) and
+// a value (the brick's device address).
+__device__ __forceinline__ uint32_t pool_slot(u64 key, uint32_t mask) {
+  u64 x = key * 0x9E3779B97F4A7C15ull;
+  x ^= x >> 29;
+  x *= 0xBF58476D1CE4E5B9ull;
+  x ^= x >> 32;
+  return (uint32_t)x & mask;
+}
+
+__global__ void k_pool_insert(const u64* __restrict__ keys, const u64* __restrict__ vals, int n, u64* tkeys, u64* tvals,
+                              uint32_t mask) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const u64 k1 = keys[i] + 1;
+    uint32_t h = pool_slot(keys[i], mask);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {  // the table has empty slots: ends within mask + 1 probes
+      const u64 prev = atomicCAS((unsigned long long*)&tkeys[h], 0ull, (unsigned long long)k1);
+      if (prev == 0 || prev == k1) {
+        tvals[h] = vals[i];
+        break;
+      }
+      h = (h + 1) & mask;
+    }
+  }
+}
+
+// out[i] = value of q[i], 0 when the key is not in the table.
+__global__ void k_pool_lookup(const u64* __restrict__ tkeys, const u64* __restrict__ tvals, uint32_t mask,
+                              const u64* __restrict__ q, long long n, u64* __restrict__ out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const u64 k1 = q[i] + 1;
+    uint32_t h = pool_slot(q[i], mask);
+    u64 v = 0;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+      const u64 t = tkeys[h];
+      if (t == k1) {
+        v = tvals[h];
+        break;
+      }
+      if (t == 0) break;
+      h = (h + 1) & mask;
+    }
+    out[i] = v;
   }
 }
 

@@ -152,6 +152,7 @@ static int noalloc_setup(int device);  // noalloc.inc
 #include "api.inc"
 #include "server.inc"
 #include "tune.inc"
+#include "pool.inc"
 #include "noalloc.inc"
 
 #ifdef IDX2G_PHASE_TIMERS

@@ -183,6 +183,11 @@ struct DecArgs {
   int bstride;           // 1 << level
   int brick[3], B3[3];   // brick dims and brick extent in samples at this level
   unsigned long long* counters;  // [0] error flags
+  // sparse-brick output (generic kernel): the extended brick after the
+  // inverse transform, k.nvox doubles per brick, and whether any block of a
+  // counted subband decoded a plane (1) or not (0); nullptr: not wanted
+  double* pool_vol;
+  uint8_t* pool_sig;
 };
 
 }  // namespace idx2g

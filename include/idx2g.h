@@ -276,6 +276,49 @@ int idx2g_server_destroy(idx2g_server* s);
  * buffers of the decode). */
 int idx2g_decode_footprint(const idx2g_dataset* ds, const idx2g_query* q, int64_t* bytes);
 
+/* ---- sparse-brick output (the reference's HashMap output mode) ----------
+ * Replaces Decode with params::out_mode::HashMap (`idx2App --brick_map`,
+ * Source/Core/idx2Decode.cpp:423-477, idx2SparseBricks.cpp:20-243): the
+ * query is decoded level by level as by idx2g_decode, and every decoded
+ * brick that is significant (a block of a detail subband, or of any subband
+ * on the coarsest level, decoded at least one bit plane;
+ * idx2Decode.cpp:205-212) is kept in HBM with its extended brick (brick + 1
+ * per axis of extent > 1) of float64 samples after the inverse transform.
+ * Bricks are addressed by the reference's brick key,
+ * (linear brick id << 4) + level (GetBrickKey, idx2Lookup.h:91-94); the pool
+ * holds a device hash table of these keys.  No output grid is written. */
+typedef struct idx2g_brick_pool idx2g_brick_pool;
+int idx2g_decode_bricks(idx2g_ctx* ctx, idx2g_dataset* ds, const idx2g_query* q, idx2g_brick_pool** out);
+/* Bricks kept on a level and bricks of the level's grid (PrintStatistics,
+ * idx2SparseBricks.cpp:49-65); levels the query did not decode keep 0. */
+int idx2g_brick_pool_count(const idx2g_brick_pool* p, int level, int64_t* kept, int64_t* total);
+/* The brick key of brick3 (brick coordinates) on a level. */
+int idx2g_brick_pool_key(const idx2g_brick_pool* p, int level, const int brick3[3], uint64_t* key);
+/* Device address of a kept brick's samples (x fastest) and its dims;
+ * IDX2G_ATTRIBUTE_NOT_FOUND when the brick is not kept. */
+int idx2g_brick_pool_find(const idx2g_brick_pool* p, int level, const int brick3[3], const double** d_samples,
+                          int dims[3]);
+/* Copy a kept brick's samples (dims from idx2g_brick_pool_find) to host
+ * (out_is_device = 0) or device memory (1). */
+int idx2g_brick_pool_copy(const idx2g_brick_pool* p, int level, const int brick3[3], double* out, int out_is_device);
+/* Device-side lookup of n brick keys (device array) through the pool's hash
+ * table: d_samples[i] = device address of the brick's samples, or NULL. */
+int idx2g_brick_pool_lookup(idx2g_ctx* ctx, const idx2g_brick_pool* p, int64_t n, const uint64_t* d_keys,
+                            const double** d_samples);
+/* Resolution of a finest-level brick: the finest level at which it or an
+ * ancestor is kept, the coarsest level when none is
+ * (ComputeBrickResolution, idx2SparseBricks.cpp:69-152). */
+int idx2g_brick_pool_resolution(const idx2g_brick_pool* p, const int brick3[3], int* level);
+/* The reference's brick file (WriteBricks and GetBrickVolume, idx2SparseBricks.cpp:157-243):
+ * dims, brick dims and finest brick counts (3 int32 each), then for every
+ * finest brick in row-major order its coordinates and sample dims (3 int32
+ * each) and its samples (float64), taken from the kept brick of its
+ * resolution.  IDX2G_FILE_CREATE_FAILED when the file cannot be created,
+ * IDX2G_ATTRIBUTE_NOT_FOUND when a finest brick has no kept brick on its
+ * path (a query that did not cover the volume). */
+int idx2g_brick_pool_write(const idx2g_brick_pool* p, const char* path);
+int idx2g_brick_pool_free(idx2g_brick_pool* p);
+
 #ifdef __cplusplus
 }
 #endif

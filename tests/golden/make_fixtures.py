@@ -132,6 +132,62 @@ def make_missing_file_case():
         json.dump(dict(source=MISSING["source"], removed=MISSING["removed"], queries=qs), f, indent=1)
 
 
+# Sparse-brick output (idx2App --brick_map, params::out_mode::HashMap): the
+# reference's "bricks" file (WriteBricks, idx2SparseBricks.cpp:207-243) and its
+# per-level significant-brick counts for decodes of fixture datasets, and of
+# a field that is zero on the slab x0 <= x < x1, whose level-0 bricks there
+# are not significant and are written from their level-1 parents.
+SPARSE = dict(dims=(80, 66, 66), dtype="float32", nlevels=2, tolerance=1e-3, brick=(32, 32, 32), bpc=8, cpf=4,
+              fpd=4, seed=11, zero_x=(30, 64))
+BRICK_MAP = [("c3d_f32_2lvl", {}), ("c3d_f32_2lvl", dict(downsampling=(1, 1, 1))), ("c3d_f64_b16", {}),
+             ("c3d_f32_64", {}), ("c2d_f32_3lvl", {}), ("c2d_f32_3lvl", dict(tolerance=1.0)),
+             ("sparse", {}), ("sparse", dict(tolerance=1e-2)), ("sparse", dict(downsampling=(1, 1, 1)))]
+
+
+def sparse_field(spec):
+    data = synth.make_field(spec["dims"], spec["dtype"], seed=spec["seed"])
+    x0, x1 = spec["zero_x"]
+    data[:, :, x0:x1] = 0
+    return data
+
+
+def make_brick_map_cases():
+    out = os.path.join(HERE, "brick_map")
+    shutil.rmtree(out, ignore_errors=True)
+    os.makedirs(out)
+    data = sparse_field(SPARSE)
+    recs = []
+    with tempfile.TemporaryDirectory() as wd:
+        rc, rdir = O.ref_encode(wd, "G", "f", data, brick=SPARSE["brick"], nlevels=SPARSE["nlevels"],
+                                tolerance=SPARSE["tolerance"], bricks_per_chunk=SPARSE["bpc"],
+                                chunks_per_file=SPARSE["cpf"], files_per_dir=SPARSE["fpd"])
+        assert rc == 0, rc
+        shutil.copytree(os.path.join(rdir, "G"), os.path.join(out, "G"))
+        for src, q in BRICK_MAP:
+            in_dir = out if src == "sparse" else os.path.join(HERE, src)
+            run = os.path.join(wd, "bm")
+            shutil.rmtree(run, ignore_errors=True)
+            os.makedirs(run)
+            cmd = [O.REF_APP, "--decode", os.path.join(in_dir, "G", "f.idx2"), "--in_dir", in_dir, "--brick_map",
+                   "--downsampling", *[str(d) for d in q.get("downsampling", (0, 0, 0))],
+                   "--tolerance", repr(q.get("tolerance", 0.0))]
+            r = subprocess.run(cmd, cwd=run, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, (src, q, r.stdout[-500:], r.stderr[-500:])
+            counts = []
+            for line in r.stdout.splitlines():
+                if "bricks significant (" in line:  # level L: K out of N bricks significant (P percent)
+                    w = line.replace(":", " ").split()
+                    counts.append([int(w[1]), int(w[2]), int(w[5])])
+            blob = open(os.path.join(run, "bricks"), "rb").read()
+            rec = {k: list(v) if isinstance(v, tuple) else v for k, v in q.items()}
+            rec.update(source=src, sha256=hashlib.sha256(blob).hexdigest(), size=len(blob), counts=counts)
+            recs.append(rec)
+    spec = {k: list(v) if isinstance(v, tuple) else v for k, v in SPARSE.items()}
+    spec["input_sha256"] = sha(data)
+    with open(os.path.join(out, "brick_map.json"), "w") as f:
+        json.dump(dict(sparse=spec, decodes=recs), f, indent=1)
+
+
 def main():
     assert O.ref_available(), "build the reference first: make -f oracle/ref.mk"
     for name, spec in CASES.items():
@@ -139,6 +195,8 @@ def main():
         print("wrote", name)
     make_missing_file_case()
     print("wrote missing_file.json")
+    make_brick_map_cases()
+    print("wrote brick_map")
 
 
 if __name__ == "__main__":
