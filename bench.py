@@ -242,13 +242,17 @@ def batched_fields(torch, ctx, seed, n_fields=64):
 
 def batched_bench(ctx, torch, seed, reps=3):
     """Encode + full decode of the 64-field mix: one call per field (the plain
-    path) against one idx2g_encode_batch + one idx2g_decode_batch."""
+    path) against one idx2g_encode_batch + one idx2g_decode_batch; the batch
+    encode as one graph launch (the default) and stepwise
+    (IDX2G_BATCH_GRAPH=0: one launch group per level, key ranges and chunk
+    layout read back by the host)."""
     from idx2_amd import api
     fields, bufs = batched_fields(torch, ctx, seed)
-    outs_by_mode = {m: [torch.empty_like(b) for b in bufs] for m in ("per_field", "batched")}
+    modes = ("per_field", "batched_stepwise", "batched")
+    outs_by_mode = {m: [torch.empty_like(b) for b in bufs] for m in modes}
     raw = sum(b.numel() * b.element_size() for b in bufs)
     res = {}
-    for mode in ("per_field", "batched"):
+    for mode in modes:
         outs = outs_by_mode[mode]
         enc, dec = [], []
         for rep in range(reps + 1):
@@ -256,6 +260,10 @@ def batched_bench(ctx, torch, seed, reps=3):
             t0 = time.perf_counter()
             if mode == "per_field":
                 dss = [api.Dataset.encode(ctx, **f) for f in fields]
+            elif mode == "batched_stepwise":
+                os.environ["IDX2G_BATCH_GRAPH"] = "0"
+                dss = api.Dataset.encode_batch(ctx, fields)
+                del os.environ["IDX2G_BATCH_GRAPH"]
             else:
                 dss = api.Dataset.encode_batch(ctx, fields)
             torch.cuda.synchronize()
@@ -278,9 +286,11 @@ def batched_bench(ctx, torch, seed, reps=3):
         res[mode] = {"encode_ms": round(e * 1e3, 3), "decode_ms": round(d * 1e3, 3),
                      "gbps": round(raw / (e + d) / 1e9, 3)}
     res["speedup"] = round(res["batched"]["gbps"] / res["per_field"]["gbps"], 3)
+    res["graph_encode_speedup"] = round(res["batched_stepwise"]["encode_ms"] / res["batched"]["encode_ms"], 3)
     # every batched decode must be bit-identical to the same field's per-field decode
     mismatched = sum(0 if torch.equal(a.view(torch.uint8), b.view(torch.uint8)) else 1
-                     for a, b in zip(outs_by_mode["per_field"], outs_by_mode["batched"]))
+                     for m in ("batched_stepwise", "batched")
+                     for a, b in zip(outs_by_mode["per_field"], outs_by_mode[m]))
     res.update({"fields": len(fields), "raw_bytes": int(raw), "chunk_bytes": int(chunk),
                 "fields_2d": sum(1 for f in fields if f["dims"][2] == 1),
                 "fields_f64": sum(1 for f in fields if f["dtype"] == "float64"),

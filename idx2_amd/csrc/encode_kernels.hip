@@ -603,6 +603,22 @@ __device__ inline int varbyte_len(u64 v) {
   return n;
 }
 
+// Keys of job J: [klo, klo + nk).  The graph form reads them from kr and
+// takes none when they exceed the job's rows (k_job_plan flags that).
+__device__ __forceinline__ void job_keys(const AsmJob& J, int* klo, int* nk) {
+  if (!J.dyn) {
+    *klo = J.klo;
+    *nk = J.nk;
+    return;
+  }
+  const int lo = J.kr[0], hi = J.kr[1];
+  const int n = hi >= lo ? hi - lo + 1 : 0;
+  *klo = lo;
+  *nk = n <= J.nk ? n : 0;
+}
+
+__device__ __forceinline__ bool job_stopped(const AsmJob& J) { return J.dyn && J.guard[2] != 0; }
+
 // Key range of each job's level from its per-brick-subband headers; start[]
 // counts workgroups per job.
 __global__ void k_key_range(const AsmJob* __restrict__ jobs, const int* __restrict__ start, int njobs) {
@@ -643,22 +659,24 @@ __global__ void k_fill_index(const AsmJob* __restrict__ jobs, const int* __restr
   const int j = job_of(start, njobs, blockIdx.x);
   const AsmJob& J = jobs[j];
   const int i = (blockIdx.x - start[j]) * blockDim.x + threadIdx.x;
-  if (i >= J.nb * J.nsb) return;
+  if (i >= J.nb * J.nsb || job_stopped(J)) return;
+  int klo, nk;
+  job_keys(J, &klo, &nk);
+  if (nk == 0) return;
   const int dense = i / J.nsb, sb = i % J.nsb;
   const SbHdr h = J.hdrs[i];
   for (int t = 0; t < h.nkeys; ++t) {
-    const int kk = h.klo + t - J.klo;
-    J.idx[((size_t)sb * J.nk + kk) * J.span + J.lin[dense]] = (uint32_t)(h.ebase + t + 1);
+    const int kk = h.klo + t - klo;
+    J.idx[((size_t)sb * nk + kk) * J.span + J.lin[dense]] = (uint32_t)(h.ebase + t + 1);
   }
 }
 
-// One workgroup per (job, subband, key, chunk): sizes of the chunk.
-__global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
-                                                    int njobs, const SbEntry* __restrict__ entries) {
-  const int j = job_of(start, njobs, blockIdx.x);
-  const AsmJob& J = jobs[j];
-  const int r = blockIdx.x - start[j];
-  const int nchunks = J.nchunks, nk = J.nk;
+// Sizes of chunk r = (sb, key - klo, chunk) of job J (record index
+// [sb][key][chunk] over the job's nk keys).  Every thread of the workgroup
+// calls it.
+__device__ __forceinline__ void chunk_size_item(const AsmJob& J, int klo, int nk, int r,
+                                                const SbEntry* __restrict__ entries) {
+  const int nchunks = J.nchunks;
   const int chunk = r % nchunks, kk = (r / nchunks) % nk, sb = r / (nchunks * nk);
   const u64 b0 = (u64)chunk * J.bpc, b1 = min(b0 + (u64)J.bpc, J.span);
   const uint32_t* row = J.idx + ((size_t)sb * nk + kk) * J.span;
@@ -706,7 +724,7 @@ __global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ j
     c.off = 0;
     c.nbricks = (uint32_t)s_cnt;
     c.sb = sb;
-    c.key = J.klo + kk;
+    c.key = klo + kk;
     c.chunk = chunk;
     c.first_brick = s_first;
     c.last_brick = s_last;
@@ -728,6 +746,14 @@ __global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ j
     }
     so[J.nsteps] = make_uint2(ov, od);
   }
+}
+
+// One workgroup per (job, subband, key, chunk): sizes of the chunk.
+__global__ void __launch_bounds__(256) k_chunk_size(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
+                                                    int njobs, const SbEntry* __restrict__ entries) {
+  const int j = job_of(start, njobs, blockIdx.x);
+  const AsmJob& J = jobs[j];
+  chunk_size_item(J, J.klo, J.nk, blockIdx.x - start[j], entries);
 }
 
 // Zero the byte ranges [dst, dst + n) of the segments (one workgroup each).
@@ -759,16 +785,15 @@ __global__ void k_scatter(const uint8_t* __restrict__ slab, const ScatterSeg* __
 // bricks, and where each brick's stream starts (decode directory: blob
 // offset + 1).  The steps' offsets inside the sections come from
 // k_chunk_size, so the steps of a chunk run in parallel.
-__global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
-                                                     int njobs, const SbEntry* __restrict__ entries,
-                                                     const uint8_t* __restrict__ arena) {
-  const int jb = job_of(start, njobs, blockIdx.x);
-  const AsmJob& J = jobs[jb];
-  const int nk = J.nk, klo = J.klo, bpc = J.bpc, nsteps = J.nsteps;
+// Work item rel of job J (keys [klo, klo + nk), chunks in blob order in
+// `order`, the job's blob at `blob`).  Every thread of the workgroup calls it.
+__device__ __forceinline__ void chunk_write_item(const AsmJob& J, int klo, int nk, int rel, const int* order,
+                                                 uint8_t* blob, const SbEntry* __restrict__ entries,
+                                                 const uint8_t* __restrict__ arena) {
+  const int bpc = J.bpc, nsteps = J.nsteps;
   const u64 span = J.span;
-  const int rel = blockIdx.x - start[jb];
   const int oi = rel / nsteps, step = rel - oi * nsteps;
-  const int ri = J.order[oi];
+  const int ri = order[oi];
   const ChunkRec c = J.recs[ri];
   const u64 b0 = (u64)c.chunk * bpc, b1 = min(b0 + (u64)bpc, span);
   const u64 s0 = b0 + (u64)step * kChunkStep;
@@ -777,7 +802,6 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
   const size_t rowi = ((size_t)c.sb * nk + (c.key - klo)) * span;
   const uint32_t* row = J.idx + rowi;
   u64* drow = J.dir + rowi;
-  uint8_t* blob = J.blob;
   uint8_t* out = blob + c.off;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int vn = varbyte_len(c.nbricks), vf = varbyte_len(c.first_brick);
@@ -924,6 +948,207 @@ __global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ 
         }
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_chunk_write(const AsmJob* __restrict__ jobs, const int* __restrict__ start,
+                                                     int njobs, const SbEntry* __restrict__ entries,
+                                                     const uint8_t* __restrict__ arena) {
+  const int jb = job_of(start, njobs, blockIdx.x);
+  const AsmJob& J = jobs[jb];
+  chunk_write_item(J, J.klo, J.nk, blockIdx.x - start[jb], J.order, J.blob, entries, arena);
+}
+
+// ---- graph form of the chunk assembly ----
+
+// Per job (one thread, in order): its key count from kr (flags bit 2, value
+// 4, in flags[2] when it exceeds the job's rows) and the first chunk-size
+// work item, cs_start[j] (njobs + 1 entries).
+__global__ void k_job_plan(const AsmJob* __restrict__ jobs, int njobs, int* cs_start, unsigned long long* flags) {
+  if (threadIdx.x != 0) return;
+  int acc = 0;
+  bool over = false;
+  const bool stopped = flags[2] != 0;
+  for (int j = 0; j < njobs; ++j) {
+    const AsmJob& J = jobs[j];
+    const int lo = J.kr[0], hi = J.kr[1];
+    over = over || (hi >= lo && hi - lo + 1 > J.nk);
+    int klo, nk;
+    job_keys(J, &klo, &nk);
+    cs_start[j] = acc;
+    if (!stopped) acc += J.nsb * nk * J.nchunks;
+  }
+  cs_start[njobs] = acc;
+  if (over) atomicOr(&flags[2], 4ull);
+}
+
+// Clear the rows of each job's stream index and directory that its keys use
+// (blockIdx.y: the job).
+__global__ void k_zero_rows(const AsmJob* __restrict__ jobs) {
+  const AsmJob& J = jobs[blockIdx.y];
+  if (job_stopped(J)) return;
+  int klo, nk;
+  job_keys(J, &klo, &nk);
+  const size_t n = (size_t)J.nsb * nk * J.span;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    J.idx[i] = 0;
+    J.dir[i] = 0;
+  }
+}
+
+// The next work item of a persistent workgroup from a queue counter (chunks
+// differ widely in size, so workgroups take items as they free up).
+__device__ __forceinline__ int next_item(int* queue) {
+  __shared__ int s_item;
+  __syncthreads();
+  if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+  __syncthreads();
+  return s_item;
+}
+
+// Chunk sizes over the work items k_job_plan counted (persistent: items of
+// similar cost, taken round robin).
+__global__ void __launch_bounds__(256) k_chunk_size_dyn(const AsmJob* __restrict__ jobs, const int* __restrict__ cs_start,
+                                                        int njobs, const SbEntry* __restrict__ entries) {
+  const int total = cs_start[njobs];
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int j = job_of(cs_start, njobs, item);
+    const AsmJob& J = jobs[j];
+    if (job_stopped(J)) return;
+    int klo, nk;
+    job_keys(J, &klo, &nk);
+    chunk_size_item(J, klo, nk, item - cs_start[j], entries);
+    __syncthreads();  // the item's LDS before the next item's
+  }
+}
+
+__device__ __forceinline__ u64 shfl_up_u64(u64 v, int d) {
+  const int lo = __shfl_up((int)(uint32_t)v, d, 64), hi = __shfl_up((int)(uint32_t)(v >> 32), d, 64);
+  return ((u64)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// One workgroup of 1024 threads per job: chunk offsets inside the job's blob
+// (prefix sums of the chunk sizes in [subband][key][chunk] order), the
+// non-empty chunks in that order (order_dyn), the blob bytes (place[1]) and
+// the number of non-empty chunks (place[2]).
+__global__ void __launch_bounds__(1024) k_blob_scan(const AsmJob* __restrict__ jobs) {
+  __shared__ u64 s_sz[16];
+  __shared__ uint32_t s_ct[16];
+  const AsmJob& J = jobs[blockIdx.x];
+  if (job_stopped(J)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int klo, nk;
+  job_keys(J, &klo, &nk);
+  const int n = J.nsb * nk * J.nchunks;
+  u64 off = 0;
+  int cnt = 0;
+  for (int i0 = 0; i0 < n; i0 += 1024) {
+    const int i = i0 + tid;
+    const uint32_t sz = i < n ? J.recs[i].size : 0u;
+    u64 x = sz;
+    uint32_t c = sz ? 1u : 0u;
+    for (int d = 1; d < 64; d <<= 1) {  // inclusive wave scans
+      const u64 t = shfl_up_u64(x, d);
+      const uint32_t tc = (uint32_t)__shfl_up((int)c, d, 64);
+      if (lane >= d) {
+        x += t;
+        c += tc;
+      }
+    }
+    if (lane == 63) {
+      s_sz[wave] = x;
+      s_ct[wave] = c;
+    }
+    __syncthreads();
+    u64 wx = 0, tot = 0;
+    uint32_t wc = 0, totc = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wave) {
+        wx += s_sz[w];
+        wc += s_ct[w];
+      }
+      tot += s_sz[w];
+      totc += s_ct[w];
+    }
+    if (i < n) {
+      J.recs[i].off = off + wx + x - sz;
+      if (sz) J.order_dyn[cnt + (int)(wc + c - 1u)] = i;
+    }
+    off += tot;
+    cnt += (int)totc;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    J.place[1] = off;
+    J.place[2] = (u64)cnt;
+  }
+}
+
+// The jobs one after the other in the batch blob (one thread, 256-byte
+// aligned: place[0]), the first chunk-writer work item of each job (wstart)
+// and its first record in chunks_out (cstart); both arrays njobs + 1 entries.
+// Flags 4 when the blob exceeds blob_cap.
+__global__ void k_blob_place(const AsmJob* __restrict__ jobs, int njobs, int* wstart, int* cstart, u64 blob_cap,
+                             unsigned long long* flags, int* queue) {
+  if (threadIdx.x != 0) return;
+  queue[0] = 0;  // k_chunk_write_dyn's next work item
+  const bool stopped = flags[2] != 0;
+  u64 base = 0;
+  int items = 0, chunks = 0;
+  for (int j = 0; j < njobs && !stopped; ++j) {
+    const AsmJob& J = jobs[j];
+    J.place[0] = base;
+    wstart[j] = items;
+    cstart[j] = chunks;
+    items += (int)J.place[2] * J.nsteps;
+    chunks += (int)J.place[2];
+    base = (base + J.place[1] + 255) & ~255ull;
+  }
+  if (stopped) {
+    for (int j = 0; j < njobs; ++j) wstart[j] = cstart[j] = 0;
+  }
+  wstart[njobs] = items;
+  cstart[njobs] = chunks;
+  if (!stopped && base > blob_cap) atomicOr(&flags[2], 4ull);
+}
+
+// The non-empty chunk records of every job, in order, into chunks_out from
+// cstart[j] on (blockIdx.x: the job).
+__global__ void k_chunks_out(const AsmJob* __restrict__ jobs, const int* __restrict__ cstart, ChunkRec* chunks_out) {
+  const AsmJob& J = jobs[blockIdx.x];
+  if (job_stopped(J)) return;
+  const int n = (int)J.place[2];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) chunks_out[cstart[blockIdx.x] + k] = J.recs[J.order_dyn[k]];
+}
+
+// Copies (and zero fills) of the segments: blockIdx.y is the segment, the
+// blockIdx.x-th of gridDim.x parts of it; 16-byte units where source and
+// destination are 16-byte aligned, bytes elsewhere.
+__global__ void __launch_bounds__(256) k_copy_segs(const CopySeg* __restrict__ segs) {
+  const CopySeg sg = segs[blockIdx.y];
+  const u64 stride = (u64)gridDim.x * blockDim.x;
+  const u64 t0 = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool vec = (((uintptr_t)sg.src | (uintptr_t)sg.dst) & 15) == 0;
+  const u64 nv = vec ? sg.n / 16 : 0;
+  for (u64 i = t0; i < nv; i += stride)
+    reinterpret_cast<uint4*>(sg.dst)[i] = sg.src ? reinterpret_cast<const uint4*>(sg.src)[i] : make_uint4(0, 0, 0, 0);
+  for (u64 i = nv * 16 + t0; i < sg.n; i += stride) sg.dst[i] = sg.src ? sg.src[i] : (uint8_t)0;
+}
+
+// The chunk writer over the work items k_blob_place counted (persistent,
+// items taken from queue[0]).
+__global__ void __launch_bounds__(256) k_chunk_write_dyn(const AsmJob* __restrict__ jobs, const int* __restrict__ wstart,
+                                                         int njobs, const SbEntry* __restrict__ entries,
+                                                         const uint8_t* __restrict__ arena, uint8_t* blob_all,
+                                                         const unsigned long long* __restrict__ flags, int* queue) {
+  if (flags[2]) return;
+  const int total = wstart[njobs];
+  for (int item = next_item(queue); item < total; item = next_item(queue)) {
+    const int jb = job_of(wstart, njobs, item);
+    const AsmJob& J = jobs[jb];
+    int klo, nk;
+    job_keys(J, &klo, &nk);
+    chunk_write_item(J, klo, nk, item - wstart[jb], J.order_dyn, blob_all + J.place[0], entries, arena);
   }
 }
 

@@ -83,6 +83,7 @@ static void level_lists(idx2g_dataset* ds) {
 static int upload_level_ids(idx2g_dataset* ds) {
   for (int l = 0; l < ds->L.nlevels; ++l) {
     LevelData& lv = ds->lv[l];
+    if (lv.d_lin) continue;
     TRY(dalloc(&lv.d_lin, lv.lin.size()));
     if (!lv.lin.empty()) HIPCHK(hipMemcpy(lv.d_lin, lv.lin.data(), lv.lin.size() * 8, hipMemcpyHostToDevice));
   }

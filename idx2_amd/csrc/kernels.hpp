@@ -100,6 +100,16 @@ struct AsmJob {
   int* kr;                   // key range out: [lo, hi]
   u64 span;
   int nb, nsb, klo, nk, bpc, nchunks, nsteps;
+  // Graph form (dyn = 1): the key range is read from kr on the device and
+  // nk is the cap of the job's key rows (idx, dir, recs and steps are sized
+  // for it); k_blob_scan writes the chunk order into order_dyn and place[1]
+  // (blob bytes), place[2] (non-empty chunks), k_blob_place place[0] (offset
+  // in the batch blob).  guard: the batch's error flags; every graph-form
+  // kernel does nothing once one is set.
+  int dyn;
+  int* order_dyn;
+  u64* place;
+  const unsigned long long* guard;
 };
 
 constexpr int kChunkStep = 256;                  // bricks per chunk-writer workgroup
@@ -110,6 +120,13 @@ constexpr int kMaxChunkSteps = 32768 / kChunkStep;  // bricks per chunk <= 32768
 struct ScatterSeg {
   u64 off, n;
   void* dst;
+};
+
+// Device-to-device copy of n bytes (src == nullptr: zero fill).
+struct CopySeg {
+  const uint8_t* src;
+  uint8_t* dst;
+  u64 n;
 };
 
 // per (subband, key, chunk) assembled chunk

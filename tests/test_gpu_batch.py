@@ -105,3 +105,82 @@ def test_batch_of_one_and_repeat(batch):
     finally:
         for a in again:
             a.free()
+
+
+def _files_of(ds, root):
+    import os
+    ds.write(root)
+    out = {}
+    for dirpath, _, names in os.walk(root):
+        for nm in names:
+            p = os.path.join(dirpath, nm)
+            out[os.path.relpath(p, root)] = open(p, "rb").read()
+    return out
+
+
+def _same_batches(ctx, fields, a, b):
+    import tempfile
+    with tempfile.TemporaryDirectory() as wd:
+        for i, (x, y) in enumerate(zip(a, b)):
+            fx = _files_of(x, f"{wd}/a{i}")
+            fy = _files_of(y, f"{wd}/b{i}")
+            assert fx == fy, fields[i]["name"]
+            assert np.array_equal(x.decode().view(np.uint8), y.decode().view(np.uint8)), fields[i]["name"]
+
+
+def test_batch_graph_equals_stepwise(batch, monkeypatch):
+    """The batch above ran as one graph launch; the same batch encoded
+    stepwise (one launch group per level, host-read key ranges and chunk
+    layout) writes the same files."""
+    from idx2_amd import api
+    ctx, fields, dss = batch
+    monkeypatch.setenv("IDX2G_BATCH_GRAPH", "0")
+    step = api.Dataset.encode_batch(ctx, fields)
+    monkeypatch.delenv("IDX2G_BATCH_GRAPH")
+    try:
+        _same_batches(ctx, fields, dss, step)
+    finally:
+        for ds in step:
+            ds.free()
+
+
+def test_graph_falls_back_when_a_level_spans_too_many_keys(batch, monkeypatch):
+    """A one-level float64 field with values near 1e-5 in its bricks of
+    x < 32 and near 1e20 in those of x >= 32, at tolerance 1e-20, spans about
+    140 bit-plane keys on its level, more than the graph form's rows hold: the
+    batch is encoded again stepwise and matches the stepwise batch."""
+    from idx2_amd import api
+    ctx, _, _ = batch
+    data = synth.make_field((64, 64, 64), "float64", seed=77)
+    data[:, :, :32] *= 1e-5
+    data[:, :, 32:] *= 1e20
+    fields = [dict(data=data, name="W", field="f", brick=(32, 32, 32), nlevels=1, tolerance=1e-20),
+              dict(data=synth.make_field((96, 80, 70), "float32", seed=78), name="N", field="f",
+                   nlevels=api.max_levels((96, 80, 70), (32, 32, 32)), tolerance=1e-4)]
+    graph = api.Dataset.encode_batch(ctx, fields)
+    monkeypatch.setenv("IDX2G_BATCH_GRAPH", "0")
+    step = api.Dataset.encode_batch(ctx, fields)
+    monkeypatch.delenv("IDX2G_BATCH_GRAPH")
+    try:
+        _same_batches(ctx, fields, graph, step)
+    finally:
+        for ds in graph + step:
+            ds.free()
+
+
+def test_arena_regrowth_keeps_results(batch, monkeypatch):
+    """Encodes that start from a 1 MiB arena overflow it, grow it and encode
+    again (stepwise for one field, as a graph for the batch): the same files
+    and decodes as without the overflow."""
+    from idx2_amd import api
+    ctx, fields, dss = batch
+    monkeypatch.setenv("IDX2G_TEST_ARENA_BYTES", str(1 << 20))
+    small = api.Dataset.encode_batch(ctx, fields[:3])
+    alone = api.Dataset.encode(ctx, **fields[0])
+    monkeypatch.delenv("IDX2G_TEST_ARENA_BYTES")
+    try:
+        _same_batches(ctx, fields[:3], small, dss[:3])
+        _same_batches(ctx, fields[:1], [alone], dss[:1])
+    finally:
+        for ds in small + [alone]:
+            ds.free()
