@@ -138,7 +138,11 @@ int idx2g_validate_params(const idx2g_encode_params* p);
  * per run, Applications/idx2App.cpp:114-168).  n fields of any shapes and
  * dtypes in one pass: each level's bricks of all fields run in the same
  * persistent brick-kernel launches and chunk assembly runs once for the
- * batch.  out[i] is byte-identical to idx2g_encode of field i alone.
+ * batch; with n > 1 the whole encode is captured into one hipGraph and
+ * launched once (key ranges and chunk layout decided on the device; a batch
+ * whose level spans more than 128 bit-plane keys is encoded stepwise; the
+ * environment variable IDX2G_BATCH_GRAPH=0 selects stepwise).  out[i] is
+ * byte-identical to idx2g_encode of field i alone.
  * inputs[i] is field i's grid in host (inputs_are_device = 0) or device
  * memory (1).  On error no dataset is returned. */
 int idx2g_encode_batch(idx2g_ctx* ctx, int n, const idx2g_encode_params* params, const void* const* inputs,
