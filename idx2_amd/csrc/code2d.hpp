@@ -23,7 +23,7 @@ struct CodeLds2 {
     double V[17 * 17 * kGroup];
     uint32_t stream[b32::kStreamWords];
   } v;
-  i64 T[kTh / 64][4][64];
+  i64 T[kTh / 64][4 * TEnc::q];  // TEnc layout
   uint32_t pool[16384];           // as CodeLds: (len | N << 8 | offset << 15) per (block, plane), f32 planes after
   uint32_t gblk[128];             // the group's block table (z = segment)
   short E[128];

@@ -54,7 +54,7 @@ struct CodeLds {
   // the arena clears what it copies), so the next subband's staging into V
   // does not wait for that copy
   uint32_t stream[kStreamWords];
-  i64 T[kTh / 64][4][64];         // row-mode transpose buffers, per wave
+  i64 T[kTh / 64][4 * TEnc::q];   // row-mode transpose buffers, per wave (TEnc layout)
   // per (block, plane): len | N << 8, stride npl; for f32 the plane words
   // follow (f64 plane words go to global scratch)
   uint32_t pool[kPoolWords];
@@ -413,7 +413,7 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
                             const Pos& p, int dense, u64* XG, const int* seg_dense = nullptr) {
   const BrickConst& k = a.k;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
-  i64* T = &C.T[p.w][0][0];
+  i64* T = &C.T[p.w][0];
   for (int g = p.w; g * 4 < nblk; g += kTh / 64) {
     const int b = g * 4 + q;
     const bool bin = b < nblk;
@@ -451,19 +451,19 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
 #pragma unroll
     for (int i0 = 0; i0 < 4; ++i0) w4[i0] = f64_to_i64(scale * v[i0]);
     if (nd >= 1) zfp_flift4(w4[0], w4[1], w4[2], w4[3]);
-    i64* Tq = T + q * 64;
+    i64* Tq = T + q * TEnc::q;
 #pragma unroll
-    for (int i0 = 0; i0 < 4; ++i0) Tq[(i1 + 4 * i2) * 4 + i0] = w4[i0];
+    for (int i0 = 0; i0 < 4; ++i0) Tq[i0 + 4 * i1 + TEnc::c * i2] = w4[i0];
     wave_sync_lds();
     {  // second axis: lane (j0, j2) lifts positions j0 + 4 i + 16 j2
       const int j0 = r & 3, j2 = r >> 2;
       i64 y4[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + 16 * j2];
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + TEnc::c * j2];
       if (nd >= 2) {
         zfp_flift4(y4[0], y4[1], y4[2], y4[3]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + 16 * j2] = y4[i];
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + TEnc::c * j2] = y4[i];
       }
     }
     wave_sync_lds();
@@ -471,11 +471,11 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
       const int j0 = r & 3, j1 = r >> 2;
       i64 y4[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * j1 + 16 * i];
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * j1 + TEnc::c * i];
       if (nd == 3) {
         zfp_flift4(y4[0], y4[1], y4[2], y4[3]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * j1 + 16 * i] = y4[i];
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * j1 + TEnc::c * i] = y4[i];
       }
     }
     wave_sync_lds();
@@ -520,8 +520,8 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
     if (narrow) {
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
-        const u64 uA = coef_u(T + (2 * pr) * 64, C, nd_s[2 * pr], lane);
-        const u64 uB = coef_u(T + (2 * pr + 1) * 64, C, nd_s[2 * pr + 1], lane);
+        const u64 uA = coef_u(T + (2 * pr) * TEnc::q, C, nd_s[2 * pr], lane);
+        const u64 uB = coef_u(T + (2 * pr + 1) * TEnc::q, C, nd_s[2 * pr + 1], lane);
         const u64 R = (uA >> 32) | (uB & 0xFFFFFFFF00000000ull);
         const u64 X = bit_transpose(R, lane);
         const int slot = 2 * pr + (lane >> 5);
@@ -534,7 +534,7 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
     } else {
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) {
-        const u64 u = coef_u(T + qq * 64, C, nd_s[qq], lane);
+        const u64 u = coef_u(T + qq * TEnc::q, C, nd_s[qq], lane);
         const u64 X = bit_transpose(u, lane);
         const int blk = g * 4 + qq;
         plane_mode<!kSeg>(X, lane, 64, blk, blk < nblk, np_s[qq], nd_s[qq], 63 - lane, C, XG, 64, e_s[qq]);
@@ -706,7 +706,7 @@ template <int DT>
 __device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, CodeLds& C, const Pos& p,
                                            int dense, u64* XG, const uint32_t* s_blocks, ArenaCursor& cur) {
   const BrickConst& k = a.k;
-  if (p.tid < 64) C.p3[p.tid] = a.perm.p3[p.tid];
+  if (p.tid < 64) C.p3[p.tid] = (unsigned char)TEnc::pos(a.perm.p3[p.tid]);  // positions in the TEnc layout
   if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
   if (p.tid < 256) {
     uint32_t e = 0;

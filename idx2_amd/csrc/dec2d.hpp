@@ -21,7 +21,7 @@ constexpr int kDecPark = kSlots * (17 * 16 + 16 * 16);
 struct DecLds2 {
   double ex[kSlots * kN];      // x = 32 column of every slot
   double V[17 * 17 * kGroup];  // one subband of the group's bricks, [segment][j][i]
-  i64 T[kTh / 64][4][64];
+  i64 T[kTh / 64][4 * TDec::q];  // TDec layout
   short E[kSlots][4][32];
   unsigned char nbps[kSlots][4][32];
   unsigned char p3[64], p2[16];
@@ -38,7 +38,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
   constexpr int npl = DT ? 64 : 32;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   const int nblk = G * nb1;
-  i64* T = &D.T[p.w][0][0];
+  i64* T = &D.T[p.w][0];
   auto xs_of = [&](int blk) {  // plane words of group block blk
     const int g = blk / nb1, lb = blk - g * nb1;
     return gm(a.xs) + ((size_t)(bi0 + s0 + g) * 8 + sb) * 128 * npl + (size_t)lb * npl;
@@ -87,7 +87,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
           const int c = lane;
           if (c < (1 << (2 * ndq))) {
             const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
-            T[sl * 64 + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+            T[sl * TDec::q + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
           }
         }
       }
@@ -101,21 +101,21 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
         const int ndq = nd_s[qq], c = lane;
         if (c < (1 << (2 * ndq))) {
           const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
-          T[qq * 64 + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+          T[qq * TDec::q + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
         }
       }
     }
     b32::wave_sync_lds();
-    i64* Tq = T + q * 64;
+    i64* Tq = T + q * TDec::q;
     {  // second axis (2D blocks: inverse lifts run y, x)
       const int j0 = r & 3, j2 = r >> 2;
       i64 y4[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + 16 * j2];
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + TDec::c * j2];
       if (nd >= 2) {
         zfp_ilift4(y4[0], y4[1], y4[2], y4[3]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + 16 * j2] = y4[i];
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + TDec::c * j2] = y4[i];
       }
     }
     b32::wave_sync_lds();
@@ -128,7 +128,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
       const int n0 = nd >= 1 ? 4 : 1;
       i64 w4[4];
 #pragma unroll
-      for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[(i1 + 4 * i2) * 4 + i0];
+      for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[i0 + 4 * i1 + TDec::c * i2];
       if (nd >= 1) zfp_ilift4(w4[0], w4[1], w4[2], w4[3]);
       const double scale = 1.0 / pow2d((63 - nd) - 1 - e);
       const int vbase = (sg * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1a : 0);

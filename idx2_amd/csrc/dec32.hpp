@@ -26,7 +26,7 @@ struct DecLds32 {
   union {
     double V[17 * 17 * 17];
   } v;
-  i64 T[kTh / 64][4][64];
+  i64 T[kTh / 64][4 * TDec::q];  // TDec layout
   short E[8][128];
   unsigned char nbps[8][128];  // planes decoded per block
   unsigned char p3[64], p2[16];
@@ -418,7 +418,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
   const int nblk = a.blk_n[sb];
   const uint32_t* blocks = s_blocks + sb * 128;
   const auto xs = gm(XS) + (size_t)sb * 128 * npl;
-  i64* T = &D.T[p.w][0][0];
+  i64* T = &D.T[p.w][0];
   for (int g = p.w; g * 4 < nblk; g += kTh / 64) {
     // this lane's row-mode block
     const int b = g * 4 + q;
@@ -460,7 +460,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
           const int c = lane;
           if (c < (1 << (2 * ndq))) {
             const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
-            T[sl * 64 + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+            T[sl * TDec::q + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
           }
         }
       }
@@ -474,21 +474,21 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
         const int ndq = nd_s[qq], c = lane;
         if (c < (1 << (2 * ndq))) {
           const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
-          T[qq * 64 + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
+          T[qq * TDec::q + dst] = (i64)((u ^ IDX2G_NBMASK) - IDX2G_NBMASK);
         }
       }
     }
     wave_sync_lds();
-    i64* Tq = T + q * 64;
+    i64* Tq = T + q * TDec::q;
     {  // third axis (inverse lifts run z, y, x)
       const int j0 = r & 3, j1 = r >> 2;
       i64 y4[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * j1 + 16 * i];
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * j1 + TDec::c * i];
       if (nd == 3) {
         zfp_ilift4(y4[0], y4[1], y4[2], y4[3]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * j1 + 16 * i] = y4[i];
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * j1 + TDec::c * i] = y4[i];
       }
     }
     wave_sync_lds();
@@ -496,11 +496,11 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       const int j0 = r & 3, j2 = r >> 2;
       i64 y4[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + 16 * j2];
+      for (int i = 0; i < 4; ++i) y4[i] = Tq[j0 + 4 * i + TDec::c * j2];
       if (nd >= 2) {
         zfp_ilift4(y4[0], y4[1], y4[2], y4[3]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + 16 * j2] = y4[i];
+        for (int i = 0; i < 4; ++i) Tq[j0 + 4 * i + TDec::c * j2] = y4[i];
       }
     }
     wave_sync_lds();
@@ -518,7 +518,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       const int n0 = nd >= 1 ? 4 : 1;
       i64 w4[4];
 #pragma unroll
-      for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[(i1 + 4 * i2) * 4 + i0];
+      for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[i0 + 4 * i1 + TDec::c * i2];
       if (nd >= 1) zfp_ilift4(w4[0], w4[1], w4[2], w4[3]);
       const double scale = 1.0 / pow2d((63 - nd) - 1 - e);
       const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);

@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
       PB = reinterpret_cast<double*>(lds_raw + b32::dec_parent_off());
       b32::load_parent_lds(a.pgrid, a.pgdims, b3, PB, p);
     }
-    if (p.tid < 64) D.p3[p.tid] = a.perm.p3[p.tid];
+    if (p.tid < 64) D.p3[p.tid] = (unsigned char)TDec::pos(a.perm.p3[p.tid]);  // positions in the TDec layout
     if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
     // exponents and decoded plane counts of the brick's blocks
     {
@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_decode_2d(const DecArgs* __restric
       D.E[s][sb][b] = e;
       D.nbps[s][sb][b] = nb;
     }
-    if (p.tid < 64) D.p3[p.tid] = a.perm.p3[p.tid];
+    if (p.tid < 64) D.p3[p.tid] = (unsigned char)TDec::pos(a.perm.p3[p.tid]);  // positions in the TDec layout
     if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
     for (int t = p.tid; t < b2d::kSlots * b32::kN; t += b2d::kTh) D.ex[t] = 0.0;
     b32::Pos pb;

@@ -557,7 +557,7 @@ __global__ void __launch_bounds__(b2d::kTh) k_encode_2d(const EncArgs* __restric
       const int s2 = t / 33, r = t - s2 * 33, hy = r & 1, jj = r >> 1, dy = 17 - hy;
       P[s_off[2 * hy] + (s2 * dy + jj) * 17 + 16] = S.ex[s2 * b32::kN + r];
     }
-    if (p.tid < 64) C.p3[p.tid] = a.perm.p3[p.tid];
+    if (p.tid < 64) C.p3[p.tid] = (unsigned char)TEnc::pos(a.perm.p3[p.tid]);  // positions in the TEnc layout
     if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
     if (p.tid < 256) {
       uint32_t e = 0;

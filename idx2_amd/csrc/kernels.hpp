@@ -5,6 +5,20 @@
 
 namespace idx2g {
 
+// Row-mode transpose buffers of the block stages: coefficient x = a + 4 b +
+// 16 c of a 4x4x4 block sits at a + 4 b + C c, and a wave's four blocks lie
+// Q words apart.  The padding (C > 16, Q > 64) spreads each lifting pass's
+// lanes over the LDS banks: a 32-lane half of the row, second-axis and
+// third-axis passes hits 24-32 distinct 8-byte bank pairs instead of 8-16.
+template <int Q, int C>
+struct TLayout {
+  static constexpr int q = Q, c = C;
+  __host__ __device__ static constexpr int pos(int x) { return (x & 15) + C * (x >> 4); }
+};
+using TEnc = TLayout<68, 17>;  // encode (the coder's LDS has room for 2 KB more)
+using TDec = TLayout<80, 19>;
+
+
 constexpr int kThreads = 256;    // threads per brick workgroup (4 waves)
 constexpr int kMaxBlocks = 125;  // 4^d blocks per subband (17^3 subband of a 32^3 brick)
 constexpr int kMaxKeys = 128;    // bit-plane keys per brick-subband
