@@ -48,7 +48,7 @@ __device__ __forceinline__ void flush_cursor(const ArenaCursor& cur, unsigned lo
 
 struct CodeLds {
   struct {
-    double V[17 * 17 * 17];
+    double V[kVXY * 17];  // one subband, kVX / kVY layout
   } v;
   // the streams of one subband; all zero between subbands (the copy into
   // the arena clears what it copies), so the next subband's staging into V
@@ -297,30 +297,30 @@ __device__ __forceinline__ void wave_sync_lds() {
 
 // ---- 1. stage subband sb of the transformed brick into V ----
 template <int HZ>
-__device__ __forceinline__ void stage_cols(const double (&c)[kN], double* V, int i, int j, int dx, int dy, int dz) {
-  double* q = V + j * dx + i;
+__device__ __forceinline__ void stage_cols(const double (&c)[kN], double* V, int i, int j, int dz) {
+  double* q = V + j * kVX + i;
 #pragma unroll
   for (int k = 0; k < 17; ++k)
-    if (k < dz) q[k * dx * dy] = c[HZ + 2 * k];
+    if (k < dz) q[k * kVXY] = c[HZ + 2 * k];
 }
 
 // Faces of subband (hx, hy, hz) from the x = 32 / y = 32 planes and the corner
 // line: x = 32 (hx = 0), y = 32 (hy = 0), corner.
 __device__ __forceinline__ void stage_faces(const Lds& S, double* V, const Pos& p, int hx, int hy, int hz) {
-  const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+  const int dz = 17 - hz;
   for (int t = p.tid; t < 17 * 16 * 2 + 17; t += kTh) {
     if (t < 272) {
       if (hx) continue;
       const int k = t >> 4, j = t & 15;
-      if (k < dz) V[(k * dy + j) * dx + 16] = S.ex[(hz + 2 * k) * 32 + hy + 2 * j];
+      if (k < dz) V[(k * kVY + j) * kVX + 16] = S.ex[(hz + 2 * k) * 32 + hy + 2 * j];
     } else if (t < 544) {
       if (hy) continue;
       const int k = (t - 272) >> 4, i = (t - 272) & 15;
-      if (k < dz) V[(k * dy + 16) * dx + i] = S.ey[(hz + 2 * k) * 32 + hx + 2 * i];
+      if (k < dz) V[(k * kVY + 16) * kVX + i] = S.ey[(hz + 2 * k) * 32 + hx + 2 * i];
     } else {
       if (hx || hy) continue;
       const int k = t - 544;
-      if (k < dz) V[(k * dy + 16) * dx + 16] = S.cc[hz + 2 * k];
+      if (k < dz) V[(k * kVY + 16) * kVX + 16] = S.cc[hz + 2 * k];
     }
   }
 }
@@ -328,8 +328,7 @@ __device__ __forceinline__ void stage_faces(const Lds& S, double* V, const Pos& 
 // z-even subband (hx, hy, 0) into V: interior from the register columns.
 __device__ __forceinline__ void stage_subband_even(const double (&c)[kN], const Lds& S, double* V, const Pos& p, int hx,
                                                    int hy) {
-  const int dx = 17 - hx, dy = 17 - hy;
-  if ((p.x & 1) == hx && (p.y & 1) == hy) stage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, 17);
+  if ((p.x & 1) == hx && (p.y & 1) == hy) stage_cols<0>(c, V, p.x >> 1, p.y >> 1, 17);
   stage_faces(S, V, p, hx, hy, 0);
 }
 
@@ -346,11 +345,10 @@ __device__ __forceinline__ void park_odd(const double (&c)[kN], double* Z1, cons
 // z-odd subband (hx, hy, 1) into V from the parked columns.
 __device__ __forceinline__ void stage_subband_odd(const double* Z1, const Lds& S, double* V, const Pos& p, int hx,
                                                   int hy) {
-  const int dx = 17 - hx, dy = 17 - hy;
   const auto src = gm(Z1) + (hx + 2 * hy) * 16 * 256;
   for (int t = p.tid; t < 16 * 256; t += kTh) {
     const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
-    V[(k * dy + j) * dx + i] = src[t];
+    V[(k * kVY + j) * kVX + i] = src[t];
   }
   stage_faces(S, V, p, hx, hy, 1);
 }
@@ -404,8 +402,9 @@ __device__ __forceinline__ void plane_mode(u64 X, int lane, int seg, int blk, bo
   }
 }
 
-// Block stage of one subband staged in C.v.V (dims dx, dy, ...): per block
-// exponent, planes and their (length, N); 4 blocks per wave.
+// Block stage of one subband staged in C.v.V (sample (i, j, k) at
+// (k * dy + j) * dx + i: dx, dy are the layout's row stride and rows per
+// plane): per block exponent, planes and their (length, N); 4 blocks per wave.
 // kSeg: the blocks belong to several bricks (block z = brick slot of the
 // group, seg_dense[z] its dense id): exponent rows and key ranges per brick.
 template <int DT, bool kSeg = false, class CL = CodeLds>
@@ -565,7 +564,7 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, int seq, 
   if (p.tid < kMaxKeys) { C.kbits[p.tid] = 0; C.kfirst[p.tid] = 0x7fffffff; }
   __syncthreads();
   IDX2G_PHASE(12);
-  block_stage<DT>(a, sb, dx, dy, blocks, nblk, C, p, dense, XG);
+  block_stage<DT>(a, sb, kVX, kVY, blocks, nblk, C, p, dense, XG);
   __syncthreads();
   IDX2G_PHASE(13);
   // ---- 4. per-key offsets over the blocks (Morton order) ----

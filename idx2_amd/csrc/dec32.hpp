@@ -24,7 +24,7 @@ namespace b32 {
 
 struct DecLds32 {
   union {
-    double V[17 * 17 * 17];
+    double V[kVXY * 17];  // one subband, kVX / kVY layout
   } v;
   i64 T[kTh / 64][4 * TDec::q];  // TDec layout
   short E[8][128];
@@ -129,30 +129,30 @@ __device__ __forceinline__ int lane_prev_i(int v) { return __builtin_amdgcn_mov_
 // Subband values of V into the register columns and the faces (inverse of
 // stage_subband).
 template <int HZ>
-__device__ __forceinline__ void unstage_cols(double (&c)[kN], const double* V, int i, int j, int dx, int dy, int dz) {
-  const double* q = V + j * dx + i;
+__device__ __forceinline__ void unstage_cols(double (&c)[kN], const double* V, int i, int j, int dz) {
+  const double* q = V + j * kVX + i;
 #pragma unroll
   for (int k = 0; k < 17; ++k)
-    if (k < dz) c[HZ + 2 * k] = q[k * dx * dy];
+    if (k < dz) c[HZ + 2 * k] = q[k * kVXY];
 }
 
 // Faces of subband (hx, hy, hz) from V back to the x = 32 / y = 32 planes and
 // the corner line.
 __device__ __forceinline__ void unstage_faces(Lds& S, const double* V, const Pos& p, int hx, int hy, int hz) {
-  const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+  const int dz = 17 - hz;
   for (int t = p.tid; t < 17 * 16 * 2 + 17; t += kTh) {
     if (t < 272) {
       if (hx) continue;
       const int k = t >> 4, j = t & 15;
-      if (k < dz) S.ex[(hz + 2 * k) * 32 + hy + 2 * j] = V[(k * dy + j) * dx + 16];
+      if (k < dz) S.ex[(hz + 2 * k) * 32 + hy + 2 * j] = V[(k * kVY + j) * kVX + 16];
     } else if (t < 544) {
       if (hy) continue;
       const int k = (t - 272) >> 4, i = (t - 272) & 15;
-      if (k < dz) S.ey[(hz + 2 * k) * 32 + hx + 2 * i] = V[(k * dy + 16) * dx + i];
+      if (k < dz) S.ey[(hz + 2 * k) * 32 + hx + 2 * i] = V[(k * kVY + 16) * kVX + i];
     } else {
       if (hx || hy) continue;
       const int k = t - 544;
-      if (k < dz) S.cc[hz + 2 * k] = V[(k * dy + 16) * dx + 16];
+      if (k < dz) S.cc[hz + 2 * k] = V[(k * kVY + 16) * kVX + 16];
     }
   }
 }
@@ -160,8 +160,7 @@ __device__ __forceinline__ void unstage_faces(Lds& S, const double* V, const Pos
 // z-even subband (hx, hy, 0) from V into the register columns.
 __device__ __forceinline__ void unstage_subband_even(double (&c)[kN], Lds& S, const double* V, const Pos& p, int hx,
                                                      int hy) {
-  const int dx = 17 - hx, dy = 17 - hy;
-  if ((p.x & 1) == hx && (p.y & 1) == hy) unstage_cols<0>(c, V, p.x >> 1, p.y >> 1, dx, dy, 17);
+  if ((p.x & 1) == hx && (p.y & 1) == hy) unstage_cols<0>(c, V, p.x >> 1, p.y >> 1, 17);
   unstage_faces(S, V, p, hx, hy, 0);
 }
 
@@ -169,11 +168,10 @@ __device__ __forceinline__ void unstage_subband_even(double (&c)[kN], Lds& S, co
 // (Z1[((hx + 2 hy) * 16 + k) * 256 + j * 16 + i] = column (2i + hx, 2j + hy)
 // at z = 2k + 1); the columns take them after every subband is decoded.
 __device__ __forceinline__ void park_subband_odd(double* Z1, Lds& S, const double* V, const Pos& p, int hx, int hy) {
-  const int dx = 17 - hx, dy = 17 - hy;
   const auto dst = gm(Z1) + (hx + 2 * hy) * 16 * 256;
   for (int t = p.tid; t < 16 * 256; t += kTh) {
     const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
-    dst[t] = V[(k * dy + j) * dx + i];
+    dst[t] = V[(k * kVY + j) * kVX + i];
   }
   unstage_faces(S, V, p, hx, hy, 1);
 }
@@ -405,9 +403,9 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
   for (int i = tid; i < NSB * MB; i += TH) gm(a.nbps)[(size_t)bi * 1024 + (i / MB) * 128 + i % MB] = (&W.nbps[0][0])[i];
 }
 
-// Step 3 for subband sb: plane words -> values in D.v.V, every sample of
-// the table's blocks (the parent's interior is copied for subband 0 below
-// the coarsest level).
+// Step 3 for subband sb: plane words -> values in D.v.V (sample (i, j, k) at
+// (k * dy + j) * dx + i), every sample of the table's blocks (the parent's
+// interior is copied for subband 0 below the coarsest level).
 template <int DT>
 __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
                                                const uint32_t* s_blocks, const u64* XS) {

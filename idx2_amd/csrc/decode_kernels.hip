@@ -316,24 +316,23 @@ __device__ __forceinline__ void decode_subband_b32(const DecArgs& a, int sb, con
                                                    const b32::Pos& p, const uint32_t* s_blocks, const u64* XS,
                                                    const double* PB = nullptr) {
   const BrickConst& k = a.k;
-  const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1], hz = k.sb[sb].f[2];
-  const int dx = 17 - hx, dy = 17 - hy, dz = 17 - hz;
+  const int dz = 17 - k.sb[sb].f[2];
   if (PB) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's parent rows are in LDS
   __syncthreads();  // V free (and every wave's parent rows in PB)
   if (!((a.mask >> sb) & 1)) {  // subband not decoded: zeros
-    for (int i = p.tid; i < dx * dy * dz; i += b32::kTh) D.v.V[i] = 0.0;
+    for (int i = p.tid; i < kVXY * dz; i += b32::kTh) D.v.V[i] = 0.0;
   } else {
     if (sb == 0 && !a.coarsest) {
       // parent's decoded data: subband-0 interior (CopyExtentGrid, Volume.h:1153-1173);
       // the blocks below write every other sample
       for (int i = p.tid; i < 16 * 16 * 16; i += b32::kTh) {
         const int x = i & 15, y = (i >> 4) & 15, z = i >> 8;
-        D.v.V[(z * 17 + y) * 17 + x] =
+        D.v.V[(z * kVY + y) * kVX + x] =
             PB ? PB[i]
                : gm(a.pgrid)[((size_t)(b3[2] * 16 + z) * a.pgdims[1] + (b3[1] * 16 + y)) * a.pgdims[0] + (b3[0] * 16 + x)];
       }
     }
-    b32::inverse_blocks<DT>(a, sb, dx, dy, D, p, s_blocks, XS);
+    b32::inverse_blocks<DT>(a, sb, kVX, kVY, D, p, s_blocks, XS);
   }
   __syncthreads();
 }

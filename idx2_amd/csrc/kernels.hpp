@@ -18,6 +18,14 @@ struct TLayout {
 using TEnc = TLayout<68, 17>;  // encode (the coder's LDS has room for 2 KB more)
 using TDec = TLayout<80, 19>;
 
+// One subband of a 32^3 brick (at most 17^3 samples) staged in LDS: sample
+// (i, j, k) at (k * kVY + j) * kVX + i, rows of 17 and planes of 17 rows for
+// every subband (16 or 17 samples per axis).  With rows of 16, the block
+// stage's row reads of an x-high subband hit 2 of 32 bank pairs per 16
+// lanes; with rows of 17 they spread over 12-16.  (Planes of 19 rows would
+// spread them further but do not fit the encode kernel's LDS.)
+constexpr int kVX = 17, kVY = 17, kVXY = kVX * kVY;
+
 
 constexpr int kThreads = 256;    // threads per brick workgroup (4 waves)
 constexpr int kMaxBlocks = 125;  // 4^d blocks per subband (17^3 subband of a 32^3 brick)
