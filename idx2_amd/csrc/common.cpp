@@ -85,7 +85,22 @@ struct DevCache {
   std::mutex mu;
   std::map<int, std::multimap<size_t, void*>> free_;      // device -> size -> block
   std::unordered_map<void*, std::pair<int, size_t>> live;  // block -> (device, size)
+  std::map<int, size_t> cached;                            // device -> bytes held free
 };
+// Give free blocks of dev back to the runtime, largest first, until at most
+// `keep` bytes stay cached (an allocation the runtime refused: only as much
+// goes back as it needs, so the rest of the cache stays warm).  Caller holds
+// c.mu.
+void trim(DevCache& c, int dev, size_t keep) {
+  auto& fl = c.free_[dev];
+  size_t& held = c.cached[dev];
+  while (held > keep && !fl.empty()) {
+    auto it = std::prev(fl.end());
+    (void)hipFree(it->second);
+    held -= it->first;
+    fl.erase(it);
+  }
+}
 DevCache& dev_cache() {
   static DevCache* c = new DevCache();  // never destroyed: blocks live until the process ends
   return *c;
@@ -163,16 +178,18 @@ hipError_t dev_alloc(void** p, size_t bytes) {
     if (it != fl.end() && it->first <= n + n / 2 + (2u << 20)) {
       *p = it->second;
       c.live[*p] = {dev, it->first};
+      c.cached[dev] -= it->first;
       fl.erase(it);
       return hipSuccess;
     }
   }
   e = hipMalloc(p, n);
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+  while (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
     (void)hipGetLastError();
-    std::lock_guard<std::mutex> g(c.mu);  // give the cached blocks back and retry
-    for (auto& kv : c.free_[dev]) (void)hipFree(kv.second);
-    c.free_[dev].clear();
+    std::lock_guard<std::mutex> g(c.mu);  // give cached blocks back, largest first, and retry
+    const size_t held = c.cached[dev];
+    if (!held) break;
+    trim(c, dev, held > n ? held - n : 0);
     e = hipMalloc(p, n);
   }
   if (e != hipSuccess) return e;
@@ -188,7 +205,9 @@ void dev_free(void* p) {
   std::lock_guard<std::mutex> g(c.mu);
   auto it = c.live.find(p);
   if (it == c.live.end()) { (void)hipFree(p); return; }
-  c.free_[it->second.first].emplace(it->second.second, p);
+  const int dev = it->second.first;
+  c.free_[dev].emplace(it->second.second, p);
+  c.cached[dev] += it->second.second;
   c.live.erase(it);
 }
 
