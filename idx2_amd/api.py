@@ -65,7 +65,8 @@ class ServerStats(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("enc_wg_per_cu", ctypes.c_int32), ("dec_wg_per_cu", ctypes.c_int32), ("walk_batch", ctypes.c_int32)]
+    _fields_ = [("enc_wg_per_cu", ctypes.c_int32), ("dec_wg_per_cu", ctypes.c_int32), ("walk_batch", ctypes.c_int32),
+                ("walk_wide_max", ctypes.c_int32)]
 
 
 EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_encode",
@@ -400,8 +401,8 @@ def autotune(ctx):
     return {k: getattr(t, k) for k, _ in Tuning._fields_}, best.value, dflt.value
 
 
-def set_tuning(device=0, enc_wg_per_cu=0, dec_wg_per_cu=0, walk_batch=0):
-    t = Tuning(enc_wg_per_cu, dec_wg_per_cu, walk_batch)
+def set_tuning(device=0, enc_wg_per_cu=0, dec_wg_per_cu=0, walk_batch=0, walk_wide_max=0):
+    t = Tuning(enc_wg_per_cu, dec_wg_per_cu, walk_batch, walk_wide_max)
     _check(lib().idx2g_set_tuning(device, ctypes.byref(t)))
 
 

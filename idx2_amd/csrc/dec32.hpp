@@ -69,10 +69,11 @@ struct BitBuf {
 };
 
 // 64 bits of the stream at bit pos (words padded by >= 16 bytes)
-__device__ __forceinline__ u64 bb_peek(const uint32_t* w32, BitBuf& B, u64 pos) {
+template <class P>
+__device__ __forceinline__ u64 bb_peek(P w32, BitBuf& B, u64 pos) {
   u64 sh = pos - B.base;
   if (sh > 64) {  // outside the buffer (also pos < base): refill at pos's word
-    const uint32_t* w = w32 + (pos >> 5);
+    const auto w = w32 + (pos >> 5);
     B.b0 = (u64)w[0] | ((u64)w[1] << 32);
     B.b1 = (u64)w[2] | ((u64)w[3] << 32);
     B.base = pos & ~31ull;
@@ -84,7 +85,8 @@ __device__ __forceinline__ u64 bb_peek(const uint32_t* w32, BitBuf& B, u64 pos) 
 // One bit plane (Decode, Zfp.h:988-1059) from 32-bit words: plane word.
 // A 64-bit window is re-read only when the next group test may need bits
 // beyond it.
-__device__ __forceinline__ u64 decode_plane_win(const uint32_t* w32, BitBuf& B, u64& pos, int& n, int nv) {
+template <class P>
+__device__ __forceinline__ u64 decode_plane_win(P w32, BitBuf& B, u64& pos, int& n, int nv) {
   u64 X = 0;
   u64 w = bb_peek(w32, B, pos);
   int avail = 64;
@@ -183,66 +185,182 @@ __device__ __forceinline__ void unpark_odd(double (&c)[kN], const double* Z1, co
 }
 
 // Steps 1-2, one workgroup per brick (k_walk): stream staging and the
-// pipelined walks.  Wave w walks subbands 2w and 2w + 1, one per 32-lane
-// segment: lane l of a segment owns key g0 - l of the current group of 32
-// keys.  The walk's shape is a template: 3D bricks (8 subbands of <= 128
-// blocks, 256 threads) and 2D bricks (4 subbands of <= 32 blocks, 128
-// threads, a small footprint so that many workgroups stay resident).
-constexpr int kWalkThreads = 256;
-constexpr int kWalkSlots = 64;                // staged streams per subband
+// pipelined walks.  Every subband with planes to decode gets a segment of
+// contiguous lanes, one lane per bit-plane key: as many lanes as the
+// subband's keys span, at most 32 (a subband spanning more keys walks them
+// in groups of 32).  The segments are packed first-fit, widest first, into
+// 64-lane bins, and the workgroup's waves take the bins in turn, so a wave
+// walks as many subbands as its lanes hold (a level-0 brick at tolerance
+// 1e-4 spans 12-22 keys per subband: two bins instead of four waves of two
+// 32-lane segments).  Lane l of a segment owns key g0 - l of the current
+// group and decodes that plane of block t - l at step t, receiving the
+// block's significance count N from lane l - 1 (the plane above) through a
+// DPP lane shift.  The walk's shape is a template: 3D bricks (8 subbands of
+// <= 128 blocks) and 2D bricks (4 subbands of <= 32 blocks).
+constexpr int kWalkSlots = 32;  // staged key streams per subband
+constexpr int kWalkCount = 64;  // keys per subband whose stream bytes are counted
 // LDS stream buffer per brick.  Bricks whose streams do not fit walk them
-// from HBM at nearly the same speed, and a small buffer leaves room for six
-// resident workgroups per CU (measured on the 1024^3 float32 decode: 4 KB
-// 24.4 ms, 20 KB 25.5 ms, 44 KB 29.3 ms).
-constexpr int kWalkStreamBytes = 4 * 1024;
+// from HBM at nearly the same speed, and a small buffer leaves room for many
+// resident workgroups per CU (measured on the 1024^3 float32 decode with
+// 256-thread walks: 4 KB 24.4 ms, 20 KB 25.5 ms, 44 KB 29.3 ms).
+constexpr int kWalkStreamBytes = 2 * 1024;
 
 template <int NSB, int MB, int TH, int SBYTES, int OCC>
 struct WalkShape {
   static constexpr int nsb = NSB;        // subbands
   static constexpr int mb = MB;          // blocks per subband (power of two)
-  static constexpr int th = TH;          // threads: two subbands per wave
+  static constexpr int th = TH;          // threads
   static constexpr int sbytes = SBYTES;  // LDS stream buffer
   // waves per SIMD the register allocation is held to (the walk is VALU
-  // bound and latency hidden by occupancy; measured on the 1024^3 float32
-  // decode: 5 waves (92 VGPRs) 22.15 ms, 6 21.59 ms, 7 (72 VGPRs, the LDS
-  // limit of the 3D shape) 21.31 ms; the 2D shape spills less at 6)
+  // bound and latency hidden by occupancy)
   static constexpr int occ = OCC;
-  static_assert(2 * (TH / 64) == NSB, "one wave per two subbands");
+  static_assert(NSB <= 8 && TH % 64 == 0 && (NSB * kWalkCount) % TH == 0, "walk shape");
 };
-using Walk3D = WalkShape<8, 128, kWalkThreads, kWalkStreamBytes, 7>;
-using Walk2D = WalkShape<4, 32, 128, 2048, 6>;
+using Walk3D = WalkShape<8, 128, 128, kWalkStreamBytes, 7>;
+using Walk3DWide = WalkShape<8, 128, 256, kWalkStreamBytes, 7>;
+using Walk2D = WalkShape<4, 32, 128, kWalkStreamBytes, 6>;
 
 template <class WS>
 struct WalkLds {
   uint32_t sbuf[WS::sbytes / 4 + 8];
-  u64 soff[WS::nsb][kWalkSlots];
-  uint32_t sstart[WS::nsb][kWalkSlots];
-  uint32_t ssize[WS::nsb][kWalkSlots];
+  uint32_t sstart[WS::nsb][kWalkSlots];  // byte offset of a staged stream in sbuf (~0u: none)
   short E[WS::nsb][WS::mb];
+  unsigned char nd[WS::nsb][WS::mb];     // block dimensions > 1 (the job's block table)
   unsigned char nbps[WS::nsb][WS::mb];
   unsigned char nsave[WS::nsb][WS::mb];
   unsigned char ssave[WS::nsb][WS::mb];
-  uint32_t wsum[WS::th / 64];
+  uint32_t wsum[8];        // per-wave sums of the stream scan, then the bins' fill levels
+  uint32_t seg_new[WS::nsb];
+  uint32_t seg[WS::nsb];  // bin | first lane << 4 | lanes << 10 | groups << 16 (~0u: not walked)
   int klo[WS::nsb], khi[WS::nsb];
-  int total, staged;
+  int total, staged, nbins;
 };
 
 // Exponent of block b of subband sb (EncodeBrickSubbandExponents layout),
 // kSkip for blocks that are not coded.
-__device__ __forceinline__ int block_exponent(const DecArgs& a, const uint32_t* s_blocks, int sb, int b, int dense,
-                                              int stride = 128) {
-  const uint32_t bw = s_blocks[sb * stride + b];
+__device__ __forceinline__ int exponent_of_word(const DecArgs& a, uint32_t bw, int sb, int dense) {
   if ((bw >> 24) & 1) return kSkip;
   const int ci = int(bw >> 25);
   const long long row = a.exp_off[sb] + ((long long)dense * a.ncoded[sb] + ci) * (a.k.dtype ? 2 : 1);
   return a.k.dtype ? int((uint32_t)a.exps[row] | ((uint32_t)a.exps[row + 1] << 8)) - 1023 : int(a.exps[row]) - 127;
 }
+__device__ __forceinline__ int block_exponent(const DecArgs& a, const uint32_t* s_blocks, int sb, int b, int dense,
+                                              int stride = 128) {
+  return exponent_of_word(a, s_blocks[sb * stride + b], sb, dense);
+}
 
-// s_blocks: the level's block tables, [subband][WS::mb].  Plane words and
-// counts go to the batch arrays in the 3D layout ([brick][8][128][npl] and
-// [brick][8][128]) whatever the shape.
+// The job's block dimensions into W.nd (once per job).
 template <class WS>
-__device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi, const uint32_t* s_blocks) {
+__device__ __forceinline__ void walk_load_blocks(const DecArgs& a, WalkLds<WS>& W) {
+  for (int i = threadIdx.x; i < WS::nsb * WS::mb; i += WS::th) {
+    const int sb = i / WS::mb, b = i % WS::mb;
+    const uint32_t bw = b < a.blk_n[sb] ? a.blocks[a.blk_off[sb] + b] : 0u;
+    W.nd[sb][b] = (unsigned char)((((bw >> 15) & 7) > 1) + (((bw >> 18) & 7) > 1) + (((bw >> 21) & 7) > 1));
+  }
+}
+
+// The pipelined walk of every segment of one bin; P: the streams (LDS
+// buffer or the chunk blob in HBM).
+template <class WS, class P>
+__device__ __forceinline__ void walk_bin(const DecArgs& a, WalkLds<WS>& W, int bi, int job, int npl, P w32) {
+  const int lane = threadIdx.x & 63;
+  const u64 lin = a.lin[bi];
+  // the arguments the steps use, read once (the plane-word stores could
+  // alias them as far as the compiler knows)
+  const int endbp = a.k.endbp, etol = a.etol;
+  const int dir_klo = a.dir_klo, dir_nk = a.dir_nk;
+  const u64 span = a.span;
+  const auto dir = gm(a.dir);
+  int sb = -1, sl = 0, wdt = 32, ngs = 0;
+#pragma unroll
+  for (int s = 0; s < WS::nsb; ++s) {
+    const uint32_t d = W.seg[s];
+    const int off = int((d >> 4) & 63), n = int((d >> 10) & 63);
+    if (d != ~0u && int(d & 15) == job && lane >= off && lane < off + n) {
+      sb = s;
+      sl = lane - off;
+      wdt = n;
+      ngs = int(d >> 16);
+    }
+  }
+  const bool sb_ok = sb >= 0;
+  const int sbi = sb_ok ? sb : 0;
+  const int klo = sb_ok ? W.klo[sbi] : 0, khi = sb_ok ? W.khi[sbi] : -1;
+  const int nblk = sb_ok ? a.blk_n[sbi] : 0;
+  const int ng = wave_max_i(ngs);
+  const auto xs = gm(a.xs) + ((size_t)bi * 8 + sbi) * 128 * npl;
+  constexpr bool kStaged = !std::is_same<P, const __attribute__((address_space(1))) uint32_t*>::value;
+  for (int gi = 0; gi < ng; ++gi) {
+    const int g0 = khi - wdt * gi;
+    const bool gact = sb_ok && gi < ngs;
+    const int key = g0 - sl;
+    const bool kact = gact && key >= klo;
+    const int kk = key - klo;
+    bool have = false;
+    u64 pos = 0;
+    if (kStaged) {
+      if (kact && kk < kWalkSlots) {
+        const uint32_t st = W.sstart[sbi][kk];
+        have = st != ~0u;
+        pos = (u64)st * 8;
+      }
+    } else if (kact && key >= dir_klo && key < dir_klo + dir_nk) {
+      const u64 d = dir[((size_t)sbi * dir_nk + (key - dir_klo)) * span + lin];
+      have = d != 0;
+      pos = dir_offset(d) * 8;
+    }
+    BitBuf B;
+    const int my_steps = gact ? nblk + min(wdt, g0 - klo + 1) : 0;
+    const int nsteps = wave_max_i(my_steps);
+    int n_out = 0, stop_out = 0;
+    int e_next = (sl == 0 && nblk > 0) ? W.E[sbi][0] : kSkip;
+    int nd_next = W.nd[sbi][0];
+    for (int t = 0; t < nsteps; ++t) {
+      const int n_left = lane_prev_i(n_out);
+      const int s_left = lane_prev_i(stop_out);
+      const int b = t - sl;
+      const bool inb = b >= 0 && b < nblk;
+      const int e = e_next;
+      const int nd = nd_next;
+      {
+        const int bn = b + 1;
+        const bool inn = bn >= 0 && bn < nblk;
+        e_next = inn ? W.E[sbi][bn] : kSkip;
+        nd_next = W.nd[sbi][inn ? bn : 0];
+      }
+      if (kact && inb && e != kSkip) {
+        const int n_in = sl == 0 ? (gi == 0 ? 0 : W.nsave[sbi][b]) : n_left;
+        const int st_in = sl == 0 ? (gi == 0 ? 0 : W.ssave[sbi][b]) : s_left;
+        const int bp = key - kBpBiasDev - e;
+        const int stop = max(64 - endbp, 57 + etol - e);
+        n_out = n_in;
+        stop_out = st_in;
+        if (bp <= 63 && bp >= stop && !st_in) {
+          if (!have) {
+            stop_out = 1;  // missing chunk: the block stops decoding planes
+          } else {
+            int n = n_in;
+            const u64 X = decode_plane_win(w32, B, pos, n, 1 << (2 * nd));
+            n_out = n;
+            xs[b * npl + (63 - bp)] = X;
+            W.nbps[sbi][b] = (unsigned char)(64 - bp);
+          }
+        } else if (bp > 63) {
+          n_out = 0;
+        }
+        if (sl == wdt - 1 || key == klo) {
+          W.nsave[sbi][b] = (unsigned char)n_out;
+          W.ssave[sbi][b] = (unsigned char)stop_out;
+        }
+      }
+    }
+  }
+}
+
+// Plane words and counts go to the batch arrays in the 3D layout
+// ([brick][8][128][npl] and [brick][8][128]) whatever the shape.
+template <class WS>
+__device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi) {
   constexpr int NSB = WS::nsb, MB = WS::mb, TH = WS::th;
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
@@ -257,7 +375,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
     W.nsave[sb][b] = 0;
     W.ssave[sb][b] = 0;
     if (sb >= k.nsb || b >= a.blk_n[sb]) continue;
-    const int e = block_exponent(a, s_blocks, sb, b, dense, MB);
+    const int e = exponent_of_word(a, a.blocks[a.blk_off[sb] + b], sb, dense);
     W.E[sb][b] = (short)e;
     if (e == kSkip) continue;
     const int stop = max(64 - k.endbp, 57 + a.etol - e);
@@ -267,11 +385,12 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
     }
   }
   __syncthreads();
-  // directory entries of the first kWalkSlots keys of each subband, regions in LDS
+  // stream regions of the first kWalkCount keys of each subband: the bytes
+  // this query reads, and the LDS offsets of the first kWalkSlots
   {
     uint32_t base = 0;
-    for (int i0 = 0; i0 < NSB * kWalkSlots; i0 += TH) {
-      const int i = i0 + tid, sb = i / kWalkSlots, kk = i % kWalkSlots;
+    for (int i0 = 0; i0 < NSB * kWalkCount; i0 += TH) {
+      const int i = i0 + tid, sb = i / kWalkCount, kk = i % kWalkCount;
       uint32_t sz = 0;
       u64 off = 0;
       if (sb < k.nsb && W.khi[sb] >= W.klo[sb]) {
@@ -281,15 +400,13 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
           if (d) { sz = dir_size(d); off = dir_offset(d); }
         }
       }
-      W.ssize[sb][kk] = sz;
-      W.soff[sb][kk] = off;
       const uint32_t reg = sz ? ((uint32_t)(off & 3) + sz + 3) & ~3u : 0u;
       const uint32_t incl = wave_scan_add(reg, lane);
       if (lane == 63) W.wsum[w] = incl;
       __syncthreads();
       uint32_t wb = base;
       for (int j = 0; j < w; ++j) wb += W.wsum[j];
-      W.sstart[sb][kk] = wb + incl - reg + (uint32_t)(off & 3);
+      if (kk < kWalkSlots) W.sstart[sb][kk] = sz ? wb + incl - reg + (uint32_t)(off & 3) : ~0u;
       for (int j = 0; j < TH / 64; ++j) base += W.wsum[j];
       __syncthreads();
     }
@@ -299,6 +416,28 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
       W.total = (int)base;
       W.staged = !wide && base <= (uint32_t)WS::sbytes;
       atomicAdd(&a.counters[1], (unsigned long long)base);  // stream bytes this query reads
+      // segments, widest first, first fit into 64-lane bins (the bins' fill
+      // levels in W.wsum, which the scan above is done with)
+      int nb = 0;
+      uint32_t placed = 0;
+      for (int i = 0; i < NSB; ++i) {
+        int s = -1, ws = 0;
+        for (int t = 0; t < NSB; ++t) {  // the widest subband not yet placed
+          const bool act = t < k.nsb && ((a.mask >> t) & 1) && W.khi[t] >= W.klo[t] && !((placed >> t) & 1);
+          const int wt = act ? min(W.khi[t] - W.klo[t] + 1, 32) : 0;
+          if (wt > ws) { ws = wt; s = t; }
+        }
+        if (s < 0) break;
+        placed |= 1u << s;
+        int bin = 0;
+        while (bin < nb && W.wsum[bin] + ws > 64) ++bin;
+        if (bin == nb) W.wsum[nb++] = 0;
+        const int ng = (W.khi[s] - W.klo[s]) / ws + 1;
+        W.seg_new[s] = (uint32_t)bin | W.wsum[bin] << 4 | (uint32_t)ws << 10 | (uint32_t)ng << 16;
+        W.wsum[bin] += ws;
+      }
+      for (int t = 0; t < NSB; ++t) W.seg[t] = ((placed >> t) & 1) ? W.seg_new[t] : ~0u;
+      W.nbins = nb;
     }
   }
   __syncthreads();
@@ -306,12 +445,13 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
   if (staged) {
     for (int i = tid; i < NSB * kWalkSlots; i += TH) {
       const int sb = i / kWalkSlots, kk = i % kWalkSlots;
-      const uint32_t sz = W.ssize[sb][kk];
-      if (!sz) continue;
-      const u64 off = W.soff[sb][kk];
+      const uint32_t st = W.sstart[sb][kk];
+      if (st == ~0u) continue;
+      const u64 d = a.dir[((size_t)sb * a.dir_nk + (W.klo[sb] + kk - a.dir_klo)) * a.span + lin];
+      const u64 off = dir_offset(d);
       const uint32_t* src = reinterpret_cast<const uint32_t*>(a.blob) + (off >> 2);
-      uint32_t* dst = W.sbuf + (W.sstart[sb][kk] >> 2);
-      const uint32_t nw = ((uint32_t)(off & 3) + sz + 3) >> 2;
+      uint32_t* dst = W.sbuf + (st >> 2);
+      const uint32_t nw = ((uint32_t)(off & 3) + dir_size(d) + 3) >> 2;
       uint32_t j = 0;
       for (; j + 8 <= nw; j += 8) {
         uint32_t t8[8];
@@ -326,81 +466,18 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
   }
   __syncthreads();
   // ---- the walks ----
-  const int seg = lane >> 5, sl = lane & 31;
-  const int sb = 2 * w + seg;
-  const bool sb_ok = sb < k.nsb && ((a.mask >> sb) & 1) && W.khi[sb] >= W.klo[sb];
-  const int klo = sb_ok ? W.klo[sb] : 0, khi = sb_ok ? W.khi[sb] : -1;
-  const int nblk = sb_ok ? a.blk_n[sb] : 0;
-  const int sbi = sb < NSB ? sb : 0;
-  const int ngr = sb_ok ? (khi - klo) / 32 + 1 : 0;
-  const int ng = max(__builtin_amdgcn_readlane(ngr, 0), __builtin_amdgcn_readlane(ngr, 32));
-  const uint32_t* blocks = s_blocks + sbi * MB;
-  const auto xs = gm(a.xs) + ((size_t)bi * 8 + sbi) * 128 * npl;
-  for (int gi = 0; gi < ng; ++gi) {
-    const int g0 = khi - 32 * gi;
-    const bool gact = sb_ok && g0 >= klo;
-    const int key = g0 - sl;
-    const bool kact = gact && key >= klo;
-    const int kk = key - klo;
-    u64 dentry = 0;
-    if (staged) {
-      if (kact && kk < kWalkSlots && W.ssize[sbi][kk]) dentry = 1;
-    } else if (kact && key >= a.dir_klo && key < a.dir_klo + a.dir_nk) {
-      dentry = gm(a.dir)[((size_t)sbi * a.dir_nk + (key - a.dir_klo)) * a.span + lin];
-    }
-    const bool have = dentry != 0;
-    const uint32_t* w32 = staged ? W.sbuf : reinterpret_cast<const uint32_t*>(a.blob);
-    u64 pos = 0;
-    if (have) pos = staged ? (u64)W.sstart[sbi][kk] * 8 : dir_offset(dentry) * 8;
-    BitBuf B;
-    const int my_steps = gact ? nblk + min(32, g0 - klo + 1) : 0;
-    const int nsteps = max(__builtin_amdgcn_readlane(my_steps, 0), __builtin_amdgcn_readlane(my_steps, 32));
-    int n_out = 0, stop_out = 0;
-    int e_next = (sl == 0 && nblk > 0) ? W.E[sbi][0] : kSkip;
-    uint32_t bw_next = blocks[0];
-    for (int t = 0; t < nsteps; ++t) {
-      const int n_left = lane_prev_i(n_out);
-      const int s_left = lane_prev_i(stop_out);
-      const int b = t - sl;
-      const bool inb = b >= 0 && b < nblk;
-      const int e = e_next;
-      const uint32_t bwc = bw_next;
-      {
-        const int bn = b + 1;
-        const bool inn = bn >= 0 && bn < nblk;
-        e_next = inn ? W.E[sbi][bn] : kSkip;
-        bw_next = blocks[inn ? bn : 0];
-      }
-      if (kact && inb && e != kSkip) {
-        const int n_in = sl == 0 ? (gi == 0 ? 0 : W.nsave[sbi][b]) : n_left;
-        const int st_in = sl == 0 ? (gi == 0 ? 0 : W.ssave[sbi][b]) : s_left;
-        const int bp = key - kBpBiasDev - e;
-        const int stop = max(64 - k.endbp, 57 + a.etol - e);
-        n_out = n_in;
-        stop_out = st_in;
-        if (bp <= 63 && bp >= stop && !st_in) {
-          if (!have) {
-            stop_out = 1;  // missing chunk: the block stops decoding planes
-          } else {
-            const int nd = (((bwc >> 15) & 7) > 1) + (((bwc >> 18) & 7) > 1) + (((bwc >> 21) & 7) > 1);
-            int n = n_in;
-            const u64 X = decode_plane_win(w32, B, pos, n, 1 << (2 * nd));
-            n_out = n;
-            xs[b * npl + (63 - bp)] = X;
-            W.nbps[sbi][b] = (unsigned char)(64 - bp);
-          }
-        } else if (bp > 63) {
-          n_out = 0;
-        }
-        if (sl == 31 || key == klo) {
-          W.nsave[sbi][b] = (unsigned char)n_out;
-          W.ssave[sbi][b] = (unsigned char)stop_out;
-        }
-      }
-    }
+  const int nbins = W.nbins;
+  for (int job = w; job < nbins; job += TH / 64) {
+    if (staged) walk_bin<WS>(a, W, bi, job, npl, static_cast<const uint32_t*>(W.sbuf));
+    else walk_bin<WS>(a, W, bi, job, npl, gm(reinterpret_cast<const uint32_t*>(a.blob)));
   }
   __syncthreads();
-  for (int i = tid; i < NSB * MB; i += TH) gm(a.nbps)[(size_t)bi * 1024 + (i / MB) * 128 + i % MB] = (&W.nbps[0][0])[i];
+  static_assert(MB % 4 == 0, "nbps rows copied in words");
+  const uint32_t* nb32 = reinterpret_cast<const uint32_t*>(&W.nbps[0][0]);
+  for (int i = tid; i < NSB * MB / 4; i += TH) {
+    const int sb = (4 * i) / MB, b = (4 * i) % MB;
+    gm(reinterpret_cast<uint32_t*>(a.nbps + (size_t)bi * 1024 + sb * 128 + b))[0] = nb32[i];
+  }
 }
 
 // Step 3 for subband sb: plane words -> values in D.v.V (sample (i, j, k) at

@@ -289,7 +289,6 @@ template <class WS>
 __global__ void __launch_bounds__(WS::th, WS::occ) k_walk(const DecArgs* __restrict__ table, const int* __restrict__ start,
                                                  int njobs) {
   __shared__ b32::WalkLds<WS> W;
-  __shared__ uint32_t s_blocks[WS::nsb * WS::mb];
   const int total = start[njobs];
   int cj = -1;
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
@@ -297,14 +296,10 @@ __global__ void __launch_bounds__(WS::th, WS::occ) k_walk(const DecArgs* __restr
     const DecArgs& a = table[j];
     if (j != cj) {
       __syncthreads();
-      for (int i = threadIdx.x; i < WS::nsb * WS::mb; i += blockDim.x) {
-        const int s = i / WS::mb, b = i % WS::mb;
-        s_blocks[i] = b < a.blk_n[s] ? a.blocks[a.blk_off[s] + b] : 0u;
-      }
+      b32::walk_load_blocks<WS>(a, W);
       cj = j;
-      __syncthreads();
     }
-    b32::walk_wg<WS>(a, W, g - start[j], s_blocks);
+    b32::walk_wg<WS>(a, W, g - start[j]);
     __syncthreads();
   }
 }

@@ -219,12 +219,18 @@ int idx2g_synth_field(idx2g_ctx* ctx, void* dev_out, const int32_t dims[3], int3
  * No counterpart in the reference.  The brick shape is fixed by the format
  * and the on-chip kernels run one 1024-thread workgroup per CU; what is left
  * to pick per device is how many workgroups of the generic kernels share a
- * CU and how many full bricks one decode walk / on-chip launch pair takes.
- * Results never depend on these (the tests check that); only time does. */
+ * CU, how many full bricks one decode walk / on-chip launch pair takes, and
+ * the decode walk's workgroup width: its subbands' key segments are packed
+ * into 64-lane bins, walked by 2-wave workgroups (many bricks resident per
+ * CU: throughput) or 4-wave ones (bins in parallel: latency, for launches
+ * with few bricks).  Results never depend on these (the tests check that);
+ * only time does. */
 typedef struct {
-  int32_t enc_wg_per_cu; /* generic encode kernel: resident workgroups per CU (0: as many as its LDS allows) */
-  int32_t dec_wg_per_cu; /* generic decode kernel: same */
-  int32_t walk_batch;    /* decode: full bricks per walk + on-chip launch pair (0: 8192) */
+  int32_t enc_wg_per_cu;  /* generic encode kernel: resident workgroups per CU (0: as many as its LDS allows) */
+  int32_t dec_wg_per_cu;  /* generic decode kernel: same */
+  int32_t walk_batch;     /* decode: full bricks per walk + on-chip launch pair (0: 32768) */
+  int32_t walk_wide_max;  /* decode: 3D walk launches of at most this many bricks use 4-wave workgroups,
+                             larger ones 2-wave workgroups (0: 8192) */
 } idx2g_tuning;
 /* Times synthetic 2D and 3D encodes/decodes on ctx's device for each
  * candidate shape, keeps the fastest for that device (process-wide) and

@@ -30,9 +30,9 @@ def _run(ctx, dims, dt, nl, brick):
     return out
 
 
-@pytest.mark.parametrize("shape", [dict(enc_wg_per_cu=1, dec_wg_per_cu=1, walk_batch=7),
-                                   dict(enc_wg_per_cu=2, dec_wg_per_cu=4, walk_batch=16),
-                                   dict(enc_wg_per_cu=8, dec_wg_per_cu=8, walk_batch=4096)])
+@pytest.mark.parametrize("shape", [dict(enc_wg_per_cu=1, dec_wg_per_cu=1, walk_batch=7, walk_wide_max=1),
+                                   dict(enc_wg_per_cu=2, dec_wg_per_cu=4, walk_batch=16, walk_wide_max=8),
+                                   dict(enc_wg_per_cu=8, dec_wg_per_cu=8, walk_batch=4096, walk_wide_max=1 << 30)])
 def test_shapes_do_not_change_results(ctx, shape):
     from idx2_amd import api
     for dims, dt, nl, brick in CASES:
@@ -50,6 +50,7 @@ def test_autotune_picks_valid_shapes(ctx):
     shapes, best, dflt = api.autotune(ctx)
     assert 0 <= shapes["enc_wg_per_cu"] <= 8 and 0 <= shapes["dec_wg_per_cu"] <= 8
     assert shapes["walk_batch"] in (0, 2048, 4096)
+    assert shapes["walk_wide_max"] in (0, 1023, 8191, 1 << 30)
     assert 0 < best <= dflt
     assert api.get_tuning(0) == shapes
     api.set_tuning(0)
