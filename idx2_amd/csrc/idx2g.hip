@@ -62,6 +62,13 @@ static void free_level(LevelData& lv) {
 }
 
 static int check_layout(const Layout& L) {
+  // block table words hold a block's origin in 5 bits per axis: subbands of
+  // at most 32 samples per axis, i.e. brick extents of at most 32
+  for (int d = 0; d < 3; ++d)
+    if (L.brick[d] > 32) {
+      set_error("brick extent %d above 32 is not supported", L.brick[d]);
+      return IDX2G_NOT_SUPPORTED;
+    }
   for (int l = 0; l < L.nlevels; ++l)
     for (int s = 0; s < L.nsb; ++s)
       if ((int)L.blocks[l][s].size() > kMaxBlocks) {

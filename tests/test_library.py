@@ -33,3 +33,19 @@ def test_synth_params_deterministic():
     for j in range(8):
         k = a[160 + 4 * j:160 + 4 * j + 3]
         assert all(float(v).is_integer() and -8 <= v <= 8 for v in k) and any(k)
+
+
+def test_brick_extents_above_32_are_refused_on_the_host():
+    """Block table words hold block origins in 5 bits per axis, so a brick
+    extent above 32 (even one whose subbands have few blocks, 64x16x16) is
+    refused with NotSupportedInVersion after the reference's own checks
+    pass; parameters the reference rejects keep the reference's code."""
+    from idx2_amd import api
+    code = {name: c for c, name in api.ERRORS.items()}
+    assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=(32, 32, 32), nlevels=2) == 0
+    assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=(16, 16, 16), nlevels=2) == 0
+    for brick in [(64, 16, 16), (16, 64, 16), (16, 16, 64), (64, 64, 64), (128, 8, 8)]:
+        assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=brick, nlevels=1) == \
+            code["NotSupportedInVersion"], brick
+    assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=(24, 16, 16), nlevels=1) == \
+        code["BrickSizeNotPowerOfTwo"]
