@@ -347,6 +347,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
   constexpr int npl = DT ? 64 : 32;
   const int total = start[njobs];
   int cj = -1;
+  IDX2G_PHASE_START();
   for (int g = blockIdx.x; g < total; g += gridDim.x) {
     const int j = job_of(start, njobs, g);
     const DecArgs& a = table[j];
@@ -387,22 +388,32 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
       D.nbps[sb][b] = a.nbps[(size_t)bi * 1024 + p.tid];
     }
     __syncthreads();
+    IDX2G_PHASE(16);
     for (int sb = 0; sb < 8; ++sb) {
       if (k.sb[sb].f[2] != 1) continue;
       decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS);
+      IDX2G_PHASE(17);
       b32::park_subband_odd(Z1, S, D.v.V, p, k.sb[sb].f[0], k.sb[sb].f[1]);
       __syncthreads();
+      IDX2G_PHASE(18);
     }
     double c[b32::kN];
     for (int sb = 0; sb < 8; ++sb) {
       if (k.sb[sb].f[2] != 0) continue;
       decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS, sb == 0 ? PB : nullptr);
+      IDX2G_PHASE(19);
       b32::unstage_subband_even(c, S, D.v.V, p, k.sb[sb].f[0], k.sb[sb].f[1]);
       __syncthreads();
+      IDX2G_PHASE(20);
     }
     b32::unpark_odd(c, Z1, p);
     b32::weigh(c, S, p, w8);
+#ifdef IDX2G_PHASE_TIMERS
+    __syncthreads();
+#endif
+    IDX2G_PHASE(21);
     b32::inverse(c, S, p);
+    IDX2G_PHASE(22);
     if (a.out) {
       // Crop(OutGrid, BrickGrid) (idx2Decode.cpp:431-445), one element per (thread, z)
       int o[2];
@@ -453,6 +464,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
       for (int z = 0; z < 32; ++z) dst[z * plane] = c[z];
     }
     __syncthreads();
+    IDX2G_PHASE(23);
   }
 }
 

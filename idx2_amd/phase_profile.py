@@ -26,6 +26,13 @@ PHASES = [
     (12, "code: stage subband into LDS"), (13, "code: block stage"), (14, "code: stream layout + emission"),
     (15, "code: copy into the arena + loop tail"),
 ]
+DEC_PHASES = [
+    (16, "decode: setup (exponents, plane counts, parent rows issued)"),
+    (17, "decode: z-odd subbands, inverse block stage"), (18, "decode: z-odd subbands parked"),
+    (19, "decode: z-even subbands, inverse block stage"), (20, "decode: z-even subbands into the columns"),
+    (21, "decode: unpark + inverse weights"), (22, "decode: inverse CDF5/3 (Z, Y, X)"),
+    (23, "decode: crop into the output"),
+]
 
 
 def main():
@@ -60,6 +67,19 @@ def main():
         ds.free()
     L.idx2g_phase_times(ticks)
     nwg = torch.cuda.get_device_properties(0).multi_processor_count
+    # one decode of the whole grid: the on-chip decode kernel's phases
+    out = torch.empty_like(x)
+    ds = api.Dataset.encode(ctx, dev_ptr=x.data_ptr(), dims=dims, dtype="float32", brick=(32, 32, 32),
+                            nlevels=3, tolerance=1e-4)
+    ds.decode(out_dev_ptr=out.data_ptr())
+    torch.cuda.synchronize()
+    dticks = (ctypes.c_ulonglong * 32)()
+    L.idx2g_phase_times(dticks)
+    ds.decode(out_dev_ptr=out.data_ptr())
+    torch.cuda.synchronize()
+    dst = ctx.stats()
+    L.idx2g_phase_times(dticks)
+    ds.free()
     tot = sum(ticks[i] for i, _ in PHASES)
     print(f"workload: {dims[0]}x{dims[1]}x{dims[2]} float32, brick 32^3, 3 levels, tolerance 1e-4")
     print(f"encode total {st['total_ms']:.3f} ms, level-0 k_encode_b32 {st['dominant_ms']:.3f} ms "
@@ -70,6 +90,16 @@ def main():
     for i, name in PHASES:
         print(f"| {name} | {ticks[i] * 10 / nwg / 1e6:.3f} | {100.0 * ticks[i] / max(tot, 1):.1f} |")
     print(f"| total | {tot * 10 / nwg / 1e6:.3f} | 100.0 |")
+    dtot = sum(dticks[i] for i, _ in DEC_PHASES)
+    print()
+    print(f"decode total {dst['total_ms']:.3f} ms (profiling build); phases of the on-chip decode kernel "
+          f"(k_decode_b32) summed over the 3 levels' launches, per workgroup")
+    print()
+    print("| phase | ms per CU | % |")
+    print("|---|---:|---:|")
+    for i, name in DEC_PHASES:
+        print(f"| {name} | {dticks[i] * 10 / nwg / 1e6:.3f} | {100.0 * dticks[i] / max(dtot, 1):.1f} |")
+    print(f"| total | {dtot * 10 / nwg / 1e6:.3f} | 100.0 |")
 
 
 if __name__ == "__main__":
