@@ -122,6 +122,84 @@ def profile_counter(kernel, key):
     return None
 
 
+COPY_MIB = 1024  # calibration copy of the HBM counters
+
+
+def pmc_probe(torch, cfg, seed):
+    """Child of measure_traffic, run under rocprofv3 --pmc: one plain device
+    copy of COPY_MIB MiB (the counters' calibration) and one encode of the
+    workload (its level-0 on-chip launch is the dominant kernel)."""
+    from idx2_amd import api
+    n = COPY_MIB << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    ctx = api.Context(0)
+    nx, ny, nz = cfg["dims"]
+    tdt = torch.float64 if cfg["dtype"] == "float64" else torch.float32
+    inp = torch.empty(nx * ny * nz, dtype=tdt, device="cuda")
+    ctx.synth_field(inp.data_ptr(), cfg["dims"], cfg["dtype"], seed)
+    torch.cuda.synchronize()
+    ds = api.Dataset.encode(ctx, dev_ptr=inp.data_ptr(), name="probe", field="f", dims=cfg["dims"],
+                            dtype=cfg["dtype"], brick=cfg["brick"], nlevels=cfg["nlevels"],
+                            tolerance=cfg["tolerance"])
+    torch.cuda.synchronize()
+    ds.free()
+    ctx.close()
+
+
+def measure_traffic(kernel, config):
+    """HBM bytes of the dominant launch, measured in this run: two
+    `rocprofv3 --pmc` passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch)
+    over a child process (pmc_probe), each scaled by what the same counter
+    reads for the child's plain copy of COPY_MIB MiB.  Returns a dict, or
+    None when the profiler is not available or a pass fails."""
+    import csv
+    import shutil
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    short = kernel.split()[0].split("::")[-1]
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as wd:
+            cmd = [prof, "--pmc", counter, "-d", wd, "-o", "run", "-f", "csv", "--", sys.executable,
+                   os.path.abspath(__file__), "--pmc-probe", "--config", config]
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=150)
+            except subprocess.TimeoutExpired:
+                log(f"traffic: rocprofv3 --pmc {counter} timed out")
+                return None
+            rows = []
+            for root, _, files in os.walk(wd):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        rows += list(csv.DictReader(open(os.path.join(root, f))))
+            if r.returncode != 0 and not rows:
+                log(f"traffic: rocprofv3 --pmc {counter} failed (rc {r.returncode})")
+                return None
+            copy, launches = None, []
+            for row in rows:
+                if row.get("Counter_Name") != counter:
+                    continue
+                name, val = row.get("Kernel_Name", ""), float(row.get("Counter_Value", 0))
+                if "copyBuffer" in name or "copy_kernel" in name or "CatArrayBatchedCopy" in name:
+                    copy = val if copy is None else max(copy, val)
+                elif short in name:
+                    launches.append(val)
+            if copy is None or not launches:
+                log(f"traffic: {counter}: no copy or no {short} dispatch in the profile")
+                return None
+            factor = copy * 1024 / (COPY_MIB << 20)  # counter bytes per copied byte
+            out[counter] = {"kib": max(launches), "copy_kib": copy, "factor": round(factor, 4),
+                            "bytes": max(launches) * 1024 / factor}
+    return {"traffic": int(out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]), "counters": out}
+
+
 def _serve_run(torch, sets, reqs, rate, slots, budget, max_out):
     """Poisson arrivals of reqs at `rate`; the client holds at most max_out
     bytes of output buffers (an overloaded server makes it wait: stalls)."""
@@ -483,6 +561,9 @@ def main():
                     help="skip the 2D float64 8192^2 slice")
     ap.add_argument("--no-end-to-end", dest="end_to_end", action="store_false",
                     help="skip the encode + file write timing")
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="skip the two rocprofv3 --pmc passes that measure the dominant launch's HBM traffic")
+    ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration under rocprofv3) and exit")
     args = ap.parse_args()
@@ -499,6 +580,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n}))
         return
+    if args.pmc_probe:
+        pmc_probe(torch, CONFIGS[args.config], args.seed)
+        return
     from idx2_amd import api
 
     ctx = api.Context(0)
@@ -512,7 +596,18 @@ def main():
                   "tuning_s": round(time.perf_counter() - t0, 2)}
     log(f"plain path: {cfg['workload']}")
     m = plain_bench(ctx, torch, cfg, args.steps, args.warmup, args.seed)
-    traffic = profile_counter(m["dom_name"], "traffic_bytes_per_launch") if args.config == "c2" else None
+    measured = None
+    if args.traffic:
+        log("HBM traffic of the dominant launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)")
+        measured = measure_traffic(m["dom_name"], args.config)
+    if measured is not None:
+        traffic = measured["traffic"]
+        traffic_source = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE over one encode of "
+                          "the workload, each scaled by the same counter over a plain %d MiB device copy" % COPY_MIB)
+    else:
+        traffic = profile_counter(m["dom_name"], "traffic_bytes_per_launch") if args.config == "c2" else None
+        traffic_source = ("profiles/k_encode_b32_1024cube_f32.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the "
+                          "same launch, copied: the profiler was not available in this run)")
     valu = profile_counter(m["dom_name"], "valu_insts_per_launch") if args.config == "c2" else None
     bricks0 = 1
     for d in range(3):
@@ -545,8 +640,8 @@ def main():
                      "frac": round(m["achieved"] / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": m["dom_name"],
                      "kernel_ms": round(m["dom_ms"], 3), "launches": m["dom_launches"],
                      "algorithmic_bytes": int(m["dom_bytes"]),
-                     "traffic_source": "profiles/k_encode_b32_1024cube_f32.md (rocprofv3 --pmc FETCH_SIZE / "
-                                       "WRITE_SIZE of the same launch, copied, not measured in this run)"},
+                     "traffic_source": traffic_source,
+                     "traffic_counters": measured["counters"] if measured is not None else None},
         "roofline_decode": {"bound": "hbm", "achieved": round(m["dec_achieved"], 2), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(m["dec_achieved"] / HBM_PEAK_GBS, 5),
                             "kernel": m["dec_name"], "kernel_ms": round(m["dec_dom_ms"], 3),
