@@ -165,7 +165,7 @@ def measure_traffic(kernel, config):
     short = kernel.split()[0].split("::")[-1]
     out = {}
     env = dict(os.environ, TMPDIR="/tmp")
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
         with tempfile.TemporaryDirectory(dir="/tmp") as wd:
             cmd = [prof, "--pmc", counter, "-d", wd, "-o", "run", "-f", "csv", "--", sys.executable,
                    os.path.abspath(__file__), "--pmc-probe", "--config", config]
@@ -191,13 +191,18 @@ def measure_traffic(kernel, config):
                     copy = val if copy is None else max(copy, val)
                 elif short in name:
                     launches.append(val)
-            if copy is None or not launches:
+            if not launches or (copy is None and counter != "SQ_INSTS_VALU"):
                 log(f"traffic: {counter}: no copy or no {short} dispatch in the profile")
                 return None
+            if counter == "SQ_INSTS_VALU":
+                out[counter] = {"wave_instructions": int(max(launches))}
+                continue
             factor = copy * 1024 / (COPY_MIB << 20)  # counter bytes per copied byte
             out[counter] = {"kib": max(launches), "copy_kib": copy, "factor": round(factor, 4),
                             "bytes": max(launches) * 1024 / factor}
-    return {"traffic": int(out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]), "counters": out}
+    return {"traffic": int(out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]),
+            "valu": out["SQ_INSTS_VALU"]["wave_instructions"],
+            "counters": {k: out[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}}
 
 
 def _serve_run(torch, sets, reqs, rate, slots, budget, max_out):
@@ -609,6 +614,10 @@ def main():
         traffic_source = ("profiles/k_encode_b32_1024cube_f32.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the "
                           "same launch, copied: the profiler was not available in this run)")
     valu = profile_counter(m["dom_name"], "valu_insts_per_launch") if args.config == "c2" else None
+    valu_source = "SQ_INSTS_VALU from profiles/k_encode_b32_1024cube_f32.md (copied)"
+    if measured is not None:
+        valu = measured["valu"]
+        valu_source = "SQ_INSTS_VALU of the same launch, rocprofv3 --pmc in this run"
     bricks0 = 1
     for d in range(3):
         bricks0 *= cfg["dims"][d] // cfg["brick"][d]
@@ -659,8 +668,8 @@ def main():
             res["compute"]["valu"] = {"bound": "valu", "insts_per_launch": valu, "achieved": round(rate, 1),
                                       "peak": 614.4, "unit": "G wave-instructions/s",
                                       "frac": round(rate / 614.4, 4),
-                                      "source": "SQ_INSTS_VALU from profiles/k_encode_b32_1024cube_f32.md; peak = 256 "
-                                                "CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction"}
+                                      "source": valu_source + "; peak = 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per "
+                                                              "wave64 instruction"}
     if tuning is not None:
         res["autotune"] = tuning
     if args.slice2d and args.config == "c2":
