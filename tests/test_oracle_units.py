@@ -208,3 +208,32 @@ def test_quantize_closed_form(oracle_lib):
     u = np.zeros(64, dtype=np.uint64)
     assert L.idx2o_block_forward(vals.ctypes.data, 3, 0, u.ctypes.data) == -127
     assert L.idx2o_block_forward(vals.ctypes.data, 3, 1, u.ctypes.data) == -1023
+
+
+def test_delayed_transpose_branch_is_unreachable():
+    """The reference's DecodeSubband (Source/Core/idx2Decode.cpp:140-203)
+    decodes a plane with zfp Decode (direct deposit) when
+    NBitPlanesDecoded = Exponent(Tolerance) - 6 - EMax + 1 is at most 8, and
+    otherwise with DecodeTest + TransposeRecursive (Zfp.h:960-985,
+    1139-1708).  With one bit plane per chunk the plane loop breaks at the
+    first plane whose RealBp = Bp + EMax is TooHighPrecision
+    (NBitPlanes - 6 > RealBp - Exponent(Tolerance) + 1), so every decoded plane
+    has EMax >= Exponent(Tolerance) - 6 and NBitPlanesDecoded <= 1: the second
+    branch never decodes a plane, which is why the oracle and the GPU path
+    implement only the direct deposit.  Checked here by running the
+    reference's loop bounds over every exponent a float32 / float64 block or
+    a tolerance can have."""
+    nbp = 64  # the first plane, Bp = 63, is in range for float32 and float64 data alike
+    for etol in range(-1080, 40):  # Exponent() is frexp's, clamped to [-1023, 1024] (Math.h:523-534)
+        for emax in range(-1080, 1030):
+            n_decoded = etol - 6 - emax + 1
+            first_plane_decoded = not (nbp - 6 > (nbp - 1 + emax) - etol + 1)
+            if first_plane_decoded:
+                assert n_decoded <= 8, (etol, emax)
+            else:  # the loop breaks before any Decode call (BitPlanesPerChunk = 1)
+                assert emax < etol - 6
+    # the tolerances of the workloads: frexp exponents as Exponent() takes them
+    import math
+    for tol in (1e-2, 1e-3, 1e-4, 1e-6, 1e-9):
+        etol = math.frexp(tol)[1]
+        assert all(etol - 6 - e + 1 <= 8 for e in range(etol - 6, 1030))
