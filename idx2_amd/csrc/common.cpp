@@ -264,20 +264,32 @@ void pinned_free(void* p) {
   c.live.erase(it);
 }
 
-int upload_block_tables(idx2g_dataset* ds) {
+void build_block_tables(idx2g_dataset* ds, std::vector<uint32_t>* t32, std::vector<uint64_t>* t64) {
   const Layout& L = ds->L;
-  std::vector<uint32_t> all;
+  const bool small = L.brick[0] <= 32 && L.brick[1] <= 32 && L.brick[2] <= 32;
+  t32->clear();
+  t64->clear();
   for (int l = 0; l < L.nlevels; ++l)
     for (int s = 0; s < L.nsb; ++s) {
-      ds->blk_off[l][s] = (int)all.size();
+      ds->blk_off[l][s] = (int)t64->size();
       int ci = 0;
       for (const BlockInfo& b : L.blocks[l][s]) {
-        all.push_back(pack_block(b.org, b.dims, b.skip, b.skip ? 0 : ci));
+        const int c = b.skip ? 0 : ci;
+        t32->push_back(small ? pack_block(b.org, b.dims, b.skip, c) : 0u);
+        t64->push_back(pack_block64(b.org, b.dims, b.skip, c));
         if (!b.skip) ++ci;
       }
     }
-  HIPCHK(dev_alloc((void**)&ds->d_blocks, all.size() * 4 + 4));
-  HIPCHK(hipMemcpy(ds->d_blocks, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+}
+
+int upload_block_tables(idx2g_dataset* ds) {
+  std::vector<uint32_t> t32;
+  std::vector<uint64_t> t64;
+  build_block_tables(ds, &t32, &t64);
+  HIPCHK(dev_alloc((void**)&ds->d_blocks, t32.size() * 4 + 4));
+  HIPCHK(hipMemcpy(ds->d_blocks, t32.data(), t32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(dev_alloc((void**)&ds->d_blocks64, t64.size() * 8 + 8));
+  HIPCHK(hipMemcpy(ds->d_blocks64, t64.data(), t64.size() * 8, hipMemcpyHostToDevice));
   return IDX2G_OK;
 }
 

@@ -60,6 +60,8 @@ __host__ __device__ inline size_t dec_lds_bytes(int npl, int mb) {
   return size_t(mb) * npl * 8 + size_t(mb) * (2 + 3) + 64;
 }
 
+// base: LDS, or the workgroup's scratch slot past the extended brick when
+// the tables do not fit LDS (bricks above 32 samples per axis)
 __device__ inline DecLds dcarve(unsigned char* base, int npl, int mb) {
   DecLds s;
   size_t o = 0;
@@ -69,13 +71,6 @@ __device__ inline DecLds dcarve(unsigned char* base, int npl, int mb) {
   s.nsave = base + o; o += mb;
   s.ssave = base + o;
   return s;
-}
-
-__device__ inline void dunpack(uint32_t w, int org[3], int dims[3], int* skip, int* ci) {
-  org[0] = w & 31; org[1] = (w >> 5) & 31; org[2] = (w >> 10) & 31;
-  dims[0] = (w >> 15) & 7; dims[1] = (w >> 18) & 7; dims[2] = (w >> 21) & 7;
-  *skip = (w >> 24) & 1;
-  *ci = int(w >> 25);
 }
 
 // Bricks of several (dataset, level) jobs per launch: table[j] describes job
@@ -92,7 +87,10 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
     const DecArgs& a = table[j];
     const BrickConst& k = a.k;
     const int npl = k.dtype ? 64 : 32;
-    DecLds S = dcarve(lds_raw, npl, a.lds_blocks);
+    DecLds S = dcarve(a.tab_off >= 0
+                          ? reinterpret_cast<unsigned char*>(a.scratch + (size_t)blockIdx.x * a.wstride + a.tab_off)
+                          : lds_raw,
+                      npl, a.lds_blocks);
     const int bi = g - start[j];
     // the extended brick: in LDS when the launch's bricks fit there (small
     // bricks, 2D), else in the workgroup's scratch slot
@@ -119,13 +117,13 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
         }
       }
       const int nblk = a.blk_n[sb];
-      const uint32_t* blocks = a.blocks + a.blk_off[sb];
+      const u64* blocks = a.blocks64 + a.blk_off[sb];
       if (tid == 0) { s_klo = 0x7fffffff; s_khi = -0x7fffffff; }
       __syncthreads();
       // exponents of the coded blocks and the key range to walk
       for (int b = tid; b < nblk; b += blockDim.x) {
         int org[3], bd[3], skip, ci;
-        dunpack(blocks[b], org, bd, &skip, &ci);
+        unpack_block64(blocks[b], org, bd, &skip, &ci);
         S.nbps[b] = 0;
         S.nsave[b] = 0;
         S.ssave[b] = 0;
@@ -175,7 +173,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
                   stop_out = true;  // missing chunk: the block stops decoding planes
                 } else {
                   int org[3], bd[3], skip, ci;
-                  dunpack(blocks[b], org, bd, &skip, &ci);
+                  unpack_block64(blocks[b], org, bd, &skip, &ci);
                   const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
                   int n = n_in;
                   const u64 X = decode_plane(a.blob, pos, n, nv);
@@ -200,7 +198,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
         const int e = S.emax[b];
         if (e == kSkip) continue;
         int org[3], bd[3], skip, ci;
-        dunpack(blocks[b], org, bd, &skip, &ci);
+        unpack_block64(blocks[b], org, bd, &skip, &ci);
         bool bypass = false;
         for (int d = 0; d < 3; ++d) bypass |= (org[d] % a.spacing[sb][d]) != 0;
         const int stop = max(64 - k.endbp, 57 + a.etol - e);

@@ -182,11 +182,24 @@ __device__ __forceinline__ double ilift_lanes(double v, int i, int D) {
   return v;
 }
 
+// Lifts the reference applies to the line at logical index i of axis ax of
+// grid g: its loops clamp the coordinates of the other two axes to the
+// storage limit M (Wavelet.h:195-199, Min(y, M.y)), so the logical lines
+// past M all lift the line at M, one after the other (on bricks whose axes
+// shrink at different rates, at the coarse levels of the extrapolation).
+// The first of them returns their count, the others 0, the rest 1.
+__device__ __forceinline__ int line_lifts(const DGrid& g, int ax, int i, const int M[3]) {
+  if (g.f[ax] + g.s[ax] * i < M[ax]) return 1;
+  if (i > 0 && g.f[ax] + g.s[ax] * (i - 1) >= M[ax]) return 0;
+  return g.d[ax] - i;
+}
+
 // One lifting pass of a whole grid along axis d.  The other two axes follow
 // the reference macros: X -> loops (Y, Z), Y -> (X, Z), Z -> (X, Y);
-// coordinates clamp to M.  Lines of at most 64 points that never reach the
-// storage limit run one point per lane, several lines per wave; others one
-// line per thread.
+// coordinates clamp to M, and a line that several logical lines clamp to is
+// lifted once per logical line, in sequence.  Lines of at most 64 points that
+// never reach the storage limit run one point per lane, several lines per
+// wave; others one line per thread.
 __device__ __forceinline__ void lift_pass(const BrickConst& k, double* F, bool fwd, int d, const DGrid& g,
                                           const int M[3]) {
   if (g.d[d] == 1) return;
@@ -199,27 +212,38 @@ __device__ __forceinline__ void lift_pass(const BrickConst& k, double* F, bool f
     const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int j0 = wave * per; j0 < nl; j0 += nw * per) {
       const int li = j0 + slot;
-      const bool act = slot < per && li < nl;
+      bool act = slot < per && li < nl;
+      int reps = 0;
       double* q = F;
       if (act) {
         const int ia = li % g.d[a], ib = li / g.d[a];
+        reps = line_lifts(g, a, ia, M) * line_lifts(g, b, ib, M);
         const int ca = min(g.f[a] + g.s[a] * ia, M[a]);
         const int cb = min(g.f[b] + g.s[b] * ib, M[b]);
         q = F + ca * st[a] + cb * st[b] + (long)(g.f[d] + g.s[d] * i) * st[d];
+        act = reps > 0;
       }
       double v = act ? *q : 0.0;
-      v = fwd ? flift_lanes(v, i, D) : ilift_lanes(v, i, D);
+      int rmax = reps;  // over the wave (the lifts' DPP reads need every lane)
+      for (int o = 32; o > 0; o >>= 1) rmax = max(rmax, __shfl_xor(rmax, o, 64));
+      for (int r = 0; r < rmax; ++r) {
+        const double u = fwd ? flift_lanes(v, i, D) : ilift_lanes(v, i, D);
+        v = r < reps ? u : v;
+      }
       if (act) *q = v;
     }
     return;
   }
   for (int li = threadIdx.x; li < nl; li += blockDim.x) {
     int ia = li % g.d[a], ib = li / g.d[a];
+    const int reps = line_lifts(g, a, ia, M) * line_lifts(g, b, ib, M);
     int ca = min(g.f[a] + g.s[a] * ia, M[a]);
     int cb = min(g.f[b] + g.s[b] * ib, M[b]);
     double* L = F + ca * st[a] + cb * st[b];
-    if (fwd) flift_line(L, st[d], g.f[d], g.d[d], g.s[d], M[d]);
-    else ilift_line(L, st[d], g.f[d], g.d[d], g.s[d], M[d]);
+    for (int r = 0; r < reps; ++r) {
+      if (fwd) flift_line(L, st[d], g.f[d], g.d[d], g.s[d], M[d]);
+      else ilift_line(L, st[d], g.f[d], g.d[d], g.s[d], M[d]);
+    }
   }
 }
 

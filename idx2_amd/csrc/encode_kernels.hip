@@ -21,9 +21,11 @@ struct EncShared {
   uint32_t total;
 };
 
-// LDS layout for one subband (dynamic shared memory): npl planes per block,
-// room for mb blocks (the launch's largest subband, at most kMaxBlocks), so
-// launches of small bricks (2D) keep several workgroups per CU.
+// Working tables for one subband: npl planes per block, room for mb blocks
+// (the launch's largest subband).  The key tables and the stream staging
+// live in LDS; the per-block tables follow them in LDS when they fit (bricks
+// of at most 32 samples per axis: several workgroups per CU for small
+// bricks), else in the workgroup's scratch slot past the extended brick.
 struct EncLds {
   u64* pw;          // [mb][npl] plane words
   uint32_t* poff;   // [mb][npl] bit offset inside the key stream
@@ -36,31 +38,31 @@ struct EncLds {
   uint32_t* stage;  // [kStageWords]
 };
 
-__host__ __device__ inline size_t enc_lds_bytes(int npl, int mb) {
-  return size_t(mb) * npl * (8 + 4 + 2) + size_t(mb) * 3 + kMaxKeys * (4 + 4 + 2) + 64 + size_t(kStageWords) * 4 + 256;
+__host__ __device__ inline size_t enc_small_bytes() { return size_t(kStageWords) * 4 + kMaxKeys * (4 + 4 + 2) + 16; }
+__host__ __device__ inline size_t enc_tab_bytes(int npl, int mb) {
+  return size_t(mb) * npl * (8 + 4 + 2) + size_t(mb) * 3 + 64;
 }
+// both parts in LDS
+__host__ __device__ inline size_t enc_lds_bytes(int npl, int mb) { return enc_small_bytes() + enc_tab_bytes(npl, mb); }
 
-__device__ inline EncLds carve(unsigned char* base, int npl, int mb) {
+// tab: the per-block tables' base, or nullptr for LDS after the small part
+__device__ inline EncLds carve(unsigned char* lds, unsigned char* tab, int npl, int mb) {
   EncLds s;
   size_t o = 0;
-  s.pw = (u64*)(base + o); o += size_t(mb) * npl * 8;
-  s.stage = (uint32_t*)(base + o); o += size_t(kStageWords) * 4;
-  s.poff = (uint32_t*)(base + o); o += size_t(mb) * npl * 4;
-  s.kbits = (uint32_t*)(base + o); o += kMaxKeys * 4;
-  s.kboff = (uint32_t*)(base + o); o += kMaxKeys * 4;
-  s.plen = (uint16_t*)(base + o); o += size_t(mb) * npl * 2;
-  s.kfirst = (short*)(base + o); o += kMaxKeys * 2;
-  s.emax = (short*)(base + o); o += size_t(mb) * 2;
-  s.np = (unsigned char*)(base + o);
+  s.stage = (uint32_t*)(lds + o); o += size_t(kStageWords) * 4;
+  s.kbits = (uint32_t*)(lds + o); o += kMaxKeys * 4;
+  s.kboff = (uint32_t*)(lds + o); o += kMaxKeys * 4;
+  s.kfirst = (short*)(lds + o); o += kMaxKeys * 2;
+  unsigned char* t = tab ? tab : lds + ((o + 15) & ~size_t(15));
+  o = 0;
+  s.pw = (u64*)(t + o); o += size_t(mb) * npl * 8;
+  s.poff = (uint32_t*)(t + o); o += size_t(mb) * npl * 4;
+  s.plen = (uint16_t*)(t + o); o += size_t(mb) * npl * 2;
+  s.emax = (short*)(t + o); o += size_t(mb) * 2;
+  s.np = (unsigned char*)(t + o);
   return s;
 }
 
-__device__ inline void unpack_block(uint32_t w, int org[3], int dims[3], int* skip, int* ci) {
-  org[0] = w & 31; org[1] = (w >> 5) & 31; org[2] = (w >> 10) & 31;
-  dims[0] = (w >> 15) & 7; dims[1] = (w >> 18) & 7; dims[2] = (w >> 21) & 7;
-  *skip = (w >> 24) & 1;
-  *ci = int(w >> 25);
-}
 
 __device__ inline void flag_error(unsigned long long* counters, unsigned long long bit) {
   atomicOr(&counters[2], bit);
@@ -102,19 +104,22 @@ __device__ void code_subbands(const EncArgs& a, const double* F, int dense, unsi
                               EncShared& sh) {
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
-  EncLds S = carve(lds_raw, npl, a.lds_blocks);
+  EncLds S = carve(lds_raw,
+                   a.tab_off >= 0 ? reinterpret_cast<unsigned char*>(a.scratch + (size_t)blockIdx.x * a.wstride + a.tab_off)
+                                  : nullptr,
+                   npl, a.lds_blocks);
   const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6, nwaves = blockDim.x >> 6;
   const int prec3[4] = {63, 62, 61, 60};
   for (int sb = 0; sb < k.nsb; ++sb) {
     const DGrid& g = k.sb[sb];
     const int nblk = a.blk_n[sb];
-    const uint32_t* blocks = a.blocks + a.blk_off[sb];
+    const u64* blocks = a.blocks64 + a.blk_off[sb];
     if (tid == 0) { sh.klo = 0x7fffffff; sh.khi = -0x7fffffff; sh.err = 0; }
     __syncthreads();
     // ---- stage A: one wave per block, one coefficient per lane ----
     for (int b = wave; b < nblk; b += nwaves) {
       int org[3], bd[3], skip, ci;
-      unpack_block(blocks[b], org, bd, &skip, &ci);
+      unpack_block64(blocks[b], org, bd, &skip, &ci);
       if (skip) {
         if (lane == 0) { S.emax[b] = kSkip; S.np[b] = 0; }
         continue;
@@ -168,7 +173,7 @@ __device__ void code_subbands(const EncArgs& a, const double* F, int dense, unsi
     for (int b = tid; b < nblk; b += blockDim.x) {
       if (S.emax[b] == kSkip) continue;
       int org[3], bd[3], skip, ci;
-      unpack_block(blocks[b], org, bd, &skip, &ci);
+      unpack_block64(blocks[b], org, bd, &skip, &ci);
       const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
       int n = 0;
       for (int p = 0; p < S.np[b]; ++p) S.plen[b * npl + p] = (uint16_t)plane_len(S.pw[b * npl + p], n, nv);
@@ -247,7 +252,7 @@ __device__ void code_subbands(const EncArgs& a, const double* F, int dense, unsi
       const int e = S.emax[b];
       if (e == kSkip) continue;
       int org[3], bd[3], skip, ci;
-      unpack_block(blocks[b], org, bd, &skip, &ci);
+      unpack_block64(blocks[b], org, bd, &skip, &ci);
       const int nv = 1 << (2 * ((bd[0] > 1) + (bd[1] > 1) + (bd[2] > 1)));
       int n = 0;
       for (int p = 0; p < S.np[b]; ++p) {

@@ -71,7 +71,8 @@ struct idx2g_dataset {
   uint64_t uid = idx2g::next_dataset_uid();
   int device = 0;
   idx2g::LevelData lv[idx2g::kMaxLevels];
-  uint32_t* d_blocks = nullptr;  // packed block tables
+  uint32_t* d_blocks = nullptr;  // packed block tables (pack_block: the on-chip kernels)
+  uint64_t* d_blocks64 = nullptr;  // the same tables as pack_block64 words (the generic kernels)
   int blk_off[idx2g::kMaxLevels][8];
   double* d_minmax = nullptr;    // per level-0 brick min/max (encode only)
   bool have_range = false;
@@ -128,6 +129,11 @@ void pinned_free(void* p);
 BrickConst make_brick_const(const Layout& L);
 PermTables make_perm_tables();
 int upload_block_tables(idx2g_dataset* ds);
+// The dataset's block tables, every level and subband (offsets into
+// ds->blk_off): 32-bit words for the on-chip kernels (zeros when a brick
+// extent is above 32, where those kernels never run) and 64-bit words for
+// the generic kernels.
+void build_block_tables(idx2g_dataset* ds, std::vector<uint32_t>* t32, std::vector<uint64_t>* t64);
 
 // zstd through the system libzstd.so.1
 int zstd_compress(const void* src, size_t n, std::vector<uint8_t>* out);

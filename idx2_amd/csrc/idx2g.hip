@@ -31,9 +31,14 @@ namespace idx2g {
 static const size_t kBlobPad = 64;
 // Chunk bytes read from files get a larger zeroed tail: a corrupted stream
 // can make a walk lane read past the stream's end, by at most one plane per
-// block (<= 193 bits: 64 verbatim, 64 positions, 65 group tests) for <= 128
-// blocks plus its 12-byte window, 3.1 KB.
-static const size_t kFileBlobPad = 4096;
+// block (<= 193 bits: 64 verbatim, 64 positions, 65 group tests) for the
+// <= 4913 blocks of a subband (bricks of up to 128 samples per axis) plus its
+// 24-byte window: 118.6 KB.
+static const size_t kFileBlobPad = size_t(128) << 10;
+// Scratch slots of a generic-kernel launch whose per-subband tables live in
+// the slots (bricks above 32 samples per axis) are held to this many bytes:
+// fewer resident workgroups, each walking more bricks.
+static const size_t kGenericScratchBytes = size_t(4) << 30;
 
 template <class T>
 static int dalloc(T** p, size_t n) {
@@ -62,17 +67,18 @@ static void free_level(LevelData& lv) {
 }
 
 static int check_layout(const Layout& L) {
-  // block table words hold a block's origin in 5 bits per axis: subbands of
-  // at most 32 samples per axis, i.e. brick extents of at most 32
+  // generic block words hold a block's origin in 8 bits per axis and the
+  // generic kernels' tables are sized for 17^3 blocks per subband: brick
+  // extents of at most 128 samples
   for (int d = 0; d < 3; ++d)
-    if (L.brick[d] > 32) {
-      set_error("brick extent %d above 32 is not supported", L.brick[d]);
+    if (L.brick[d] > kMaxBrickExtent) {
+      set_error("brick extent %d above %d is not supported", L.brick[d], kMaxBrickExtent);
       return IDX2G_NOT_SUPPORTED;
     }
   for (int l = 0; l < L.nlevels; ++l)
     for (int s = 0; s < L.nsb; ++s)
-      if ((int)L.blocks[l][s].size() > kMaxBlocks) {
-        set_error("brick size gives %zu blocks per subband (max %d)", L.blocks[l][s].size(), kMaxBlocks);
+      if ((int)L.blocks[l][s].size() > kMaxGenericBlocks) {
+        set_error("brick size gives %zu blocks per subband (max %d)", L.blocks[l][s].size(), kMaxGenericBlocks);
         return IDX2G_NOT_SUPPORTED;
       }
   return IDX2G_OK;

@@ -33,10 +33,25 @@ constexpr int kMaxKeys = 128;    // bit-plane keys per brick-subband
 constexpr int kStageWords = 4096;  // LDS staging for one subband's streams (16 KB)
 constexpr short kSkip = -32768;
 
-// block table entry: org(5 bits x3) | dims(3 bits x3) << 15 | skip << 24 | coded index << 25
+// block table entry of the on-chip kernels (bricks of at most 32 samples
+// per axis): org(5 bits x3) | dims(3 bits x3) << 15 | skip << 24 | coded index << 25
 __host__ __device__ inline uint32_t pack_block(const int org[3], const int dims[3], int skip, int ci) {
   return uint32_t(org[0] | (org[1] << 5) | (org[2] << 10) | (dims[0] << 15) | (dims[1] << 18) |
                   (dims[2] << 21) | (skip << 24)) | (uint32_t(ci) << 25);
+}
+// block table entry of the generic kernels (any brick size this build
+// takes): org(8 bits x3) | dims(3 bits x3) << 24 | skip << 33 | coded index << 34
+constexpr int kMaxBrickExtent = 128;      // samples per axis (subbands of <= 65: 17 blocks per axis)
+constexpr int kMaxGenericBlocks = 4913;   // 4^d blocks per subband (17^3)
+__host__ __device__ inline u64 pack_block64(const int org[3], const int dims[3], int skip, int ci) {
+  return (u64)org[0] | ((u64)org[1] << 8) | ((u64)org[2] << 16) | ((u64)dims[0] << 24) | ((u64)dims[1] << 27) |
+         ((u64)dims[2] << 30) | ((u64)skip << 33) | ((u64)ci << 34);
+}
+__host__ __device__ inline void unpack_block64(u64 w, int org[3], int dims[3], int* skip, int* ci) {
+  org[0] = int(w & 255); org[1] = int((w >> 8) & 255); org[2] = int((w >> 16) & 255);
+  dims[0] = int((w >> 24) & 7); dims[1] = int((w >> 27) & 7); dims[2] = int((w >> 30) & 7);
+  *skip = int((w >> 33) & 1);
+  *ci = int(w >> 34);
 }
 
 // Chunk directory entry of one (subband, key, brick) stream in a level's
@@ -73,14 +88,17 @@ struct EncArgs {
   int per;               // 2D kernels: bricks per workgroup iteration (1..32)
   double* scratch;       // [gridDim.x][wstride] (generic kernel: work volume; b32: parked z-odd half)
   int wstride;           // doubles per workgroup slot of scratch
-  int lds_blocks;        // generic kernel: blocks per subband its LDS layout holds (launch-wide)
+  int lds_blocks;        // generic kernel: blocks per subband its tables hold (launch-wide)
   int lds_brick;         // generic kernel: byte offset of the extended brick in LDS, -1: in scratch
+  int tab_off;           // generic kernel: per-subband tables at this double offset of the workgroup's
+                         // scratch slot (past the extended brick), -1: in LDS
   // next level's grid (non-coarsest): subband-0 interiors at brick * sbne
   double* parents;
   int pdims[3];
   int sbne[3];           // non-extended subband-0 dims
   // block tables
-  const uint32_t* blocks;
+  const uint32_t* blocks;  // on-chip kernels (pack_block)
+  const u64* blocks64;     // generic kernel (pack_block64)
   int blk_off[8], blk_n[8], ncoded[8];
   double wfwd[8];
   int etol;
@@ -191,8 +209,10 @@ struct DecArgs {
   int per;               // 2D kernels: bricks per workgroup iteration (1..32)
   double* scratch;       // per workgroup slot: work volume (generic kernel), parked z-odd half (b32)
   int wstride;           // doubles per workgroup slot
-  int lds_blocks;        // generic kernel: blocks per subband its LDS layout holds (launch-wide)
+  int lds_blocks;        // generic kernel: blocks per subband its tables hold (launch-wide)
   int lds_brick;         // generic kernel: byte offset of the extended brick in LDS, -1: in scratch
+  int tab_off;           // generic kernel: per-subband tables at this double offset of the workgroup's
+                         // scratch slot (past the extended brick), -1: in LDS
   u64* xs;               // b32: plane words of the batch [brick][subband][128 blocks][npl]
   uint8_t* nbps;         // b32: planes decoded per block [brick][subband][128]
   // Level grids hold only the query box's bricks: pgrid / cgrid are shifted
@@ -204,7 +224,8 @@ struct DecArgs {
   int cgdims[3];
   int cgend[3];          // global end (exclusive) of the cgrid box per axis
   int sbne[3];
-  const uint32_t* blocks;
+  const uint32_t* blocks;  // on-chip kernels (pack_block)
+  const u64* blocks64;     // generic kernel (pack_block64)
   int blk_off[8], blk_n[8], ncoded[8];
   double winv[8];
   const uint8_t* blob;

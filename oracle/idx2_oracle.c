@@ -1022,7 +1022,9 @@ static u32 split_by2(u32 x)
 }
 static u32 morton3(u32 x, u32 y, u32 z) { return split_by2(x) | (split_by2(y) << 1) | (split_by2(z) << 2); }
 
-typedef struct { int n; int org[125][3]; int dim[125][3]; } blocklist_t;
+/* at most 17^3 blocks per subband: bricks of up to 128 samples per axis */
+#define OMAXBLOCKS 4913
+typedef struct { int n; int org[OMAXBLOCKS][3]; int dim[OMAXBLOCKS][3]; } blocklist_t;
 
 /* Blocks of a subband in Morton order (Zfp loop of idx2Encode.cpp:333-350) */
 static void block_list(const int sbdims[3], blocklist_t* L)
@@ -1108,7 +1110,7 @@ typedef struct {
   int bricks3[MAXL][3];
   /* per subband pass */
   struct { u32 block; int bpkey; } lastsig[256]; int nlastsig;
-  short sbexps[125]; int nsbexps;
+  short sbexps[OMAXBLOCKS]; int nsbexps;
   ostat_t st_bpchunk, st_deltas, st_bsizes, st_chunksizes, st_caddr_u, st_caddr_c;
   ostat_t st_exp_u, st_exp_c, st_expsizes, st_eaddr_u, st_eaddr_c;
   int err;

@@ -8,7 +8,7 @@ a set of decode queries the sha256 and a few samples of the reference's
 decoded grid are recorded.
 
 Run in the container that has the reference sources:
-    make -f oracle/ref.mk && python tests/golden/make_fixtures.py
+    make -f oracle/ref.mk && python tests/golden/make_fixtures.py [--only CASE ...]
 """
 import hashlib
 import json
@@ -40,6 +40,13 @@ CASES = {
                     brick=(32, 32, 1), bpc=4096, cpf=64, fpd=64, seed=1),
     "c2d_f32_3lvl": dict(dims=(300, 260, 1), dtype="float32", nlevels=3, tolerance=1e-4,
                          brick=(32, 32, 1), bpc=4, cpf=2, fpd=4, seed=2),
+    # bricks above 32 samples per axis (the generic kernels, 64-bit block words)
+    "c3d_f32_b64": dict(dims=(128, 80, 70), dtype="float32", nlevels=1, tolerance=1e-3,
+                        brick=(64, 64, 64), bpc=4, cpf=2, fpd=2, seed=13),
+    "c3d_f64_b64x32x16": dict(dims=(130, 66, 34), dtype="float64", nlevels=2, tolerance=1e-4,
+                              brick=(64, 32, 16), bpc=8, cpf=4, fpd=4, seed=17),
+    "c2d_f64_b64": dict(dims=(300, 260, 1), dtype="float64", nlevels=2, tolerance=1e-4,
+                        brick=(64, 64, 1), bpc=4, cpf=4, fpd=4, seed=19),
 }
 
 
@@ -190,13 +197,16 @@ def make_brick_map_cases():
 
 def main():
     assert O.ref_available(), "build the reference first: make -f oracle/ref.mk"
+    only = sys.argv[sys.argv.index("--only") + 1:] if "--only" in sys.argv else None
     for name, spec in CASES.items():
-        make_case(name, spec)
-        print("wrote", name)
-    make_missing_file_case()
-    print("wrote missing_file.json")
-    make_brick_map_cases()
-    print("wrote brick_map")
+        if only is None or name in only:
+            make_case(name, spec)
+            print("wrote", name)
+    if only is None:
+        make_missing_file_case()
+        print("wrote missing_file.json")
+        make_brick_map_cases()
+        print("wrote brick_map")
 
 
 if __name__ == "__main__":

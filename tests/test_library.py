@@ -35,17 +35,19 @@ def test_synth_params_deterministic():
         assert all(float(v).is_integer() and -8 <= v <= 8 for v in k) and any(k)
 
 
-def test_brick_extents_above_32_are_refused_on_the_host():
-    """Block table words hold block origins in 5 bits per axis, so a brick
-    extent above 32 (even one whose subbands have few blocks, 64x16x16) is
-    refused with NotSupportedInVersion after the reference's own checks
-    pass; parameters the reference rejects keep the reference's code."""
+def test_brick_extents_up_to_128_validate_like_the_reference():
+    """Generic block words hold block origins in 8 bits per axis and the
+    generic kernels' tables 17^3 blocks per subband, so brick extents up to
+    128 are taken (also anisotropic ones such as 64x16x16, whose subbands
+    have few blocks but origins past 31); an extent of 256 is refused with
+    NotSupportedInVersion after the reference's own checks pass, and
+    parameters the reference rejects keep the reference's code."""
     from idx2_amd import api
     code = {name: c for c, name in api.ERRORS.items()}
     assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=(32, 32, 32), nlevels=2) == 0
-    assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=(16, 16, 16), nlevels=2) == 0
-    for brick in [(64, 16, 16), (16, 64, 16), (16, 16, 64), (64, 64, 64), (128, 8, 8)]:
-        assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=brick, nlevels=1) == \
-            code["NotSupportedInVersion"], brick
+    for brick in [(64, 16, 16), (16, 64, 16), (16, 16, 64), (64, 64, 64), (128, 8, 8), (128, 128, 128)]:
+        assert api.validate_params(dims=(256, 256, 256), dtype="float32", brick=brick, nlevels=1) == 0, brick
+    assert api.validate_params(dims=(512, 64, 64), dtype="float32", brick=(256, 16, 16), nlevels=1) == \
+        code["NotSupportedInVersion"]
     assert api.validate_params(dims=(128, 128, 128), dtype="float32", brick=(24, 16, 16), nlevels=1) == \
         code["BrickSizeNotPowerOfTwo"]
