@@ -27,8 +27,9 @@ struct DecLds2 {
   unsigned char p3[64], p2[16];
 };
 
-// Subband sb of the group's G bricks (slots s0 ..): plane words of slot s at
-// a.xs + ((bi0 + s) * 8 + sb) * 128 * npl (the walk's layout); values to
+// Subband sb of the group's G bricks (slots s0 ..): plane words of slot s in
+// the region a.xs + ((bi0 + s) * 8 + sb) * xs_region(npl) (the walk's
+// layout, b32::xs_at); values to
 // P[(g * dy + j) * dx + i] for g = slot - s0.
 template <int DT>
 __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int s0, int G, int dx, int dy, DecLds2& D,
@@ -39,9 +40,9 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   const int nblk = G * nb1;
   i64* T = &D.T[p.w][0];
-  auto xs_of = [&](int blk) {  // plane words of group block blk
+  auto xs_word = [&](int blk, int pp) {  // plane word pp of group block blk
     const int g = blk / nb1, lb = blk - g * nb1;
-    return gm(a.xs) + ((size_t)(bi0 + s0 + g) * 8 + sb) * 128 * npl + (size_t)lb * npl;
+    return gm(a.xs)[((size_t)(bi0 + s0 + g) * 8 + sb) * b32::xs_region(npl) + b32::xs_at(lb, pp, npl)];
   };
   auto nbps_of = [&](int blk) -> int {
     const int g = blk / nb1, lb = blk - g * nb1;
@@ -78,7 +79,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
         const int slot = 2 * pr + (lane >> 5);
         const int blk = g4 * 4 + slot;
         const int pp = 31 - (lane & 31);
-        const u64 X = (blk < nblk && pp < nbps_of(blk)) ? xs_of(blk)[pp] : 0ull;
+        const u64 X = (blk < nblk && pp < nbps_of(blk)) ? xs_word(blk, pp) : 0ull;
         const u64 R = b32::bit_transpose(X, lane);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -96,7 +97,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
       for (int qq = 0; qq < 4; ++qq) {
         const int blk = g4 * 4 + qq;
         const int pp = 63 - lane;
-        const u64 X = (blk < nblk && pp < nbps_of(blk)) ? xs_of(blk)[pp] : 0ull;
+        const u64 X = (blk < nblk && pp < nbps_of(blk)) ? xs_word(blk, pp) : 0ull;
         const u64 u = b32::bit_transpose(X, lane);
         const int ndq = nd_s[qq], c = lane;
         if (c < (1 << (2 * ndq))) {

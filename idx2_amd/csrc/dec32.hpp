@@ -32,6 +32,16 @@ struct DecLds32 {
   unsigned char p3[64], p2[16];
 };
 
+// Plane words of one (brick, subband) in the walk's batch array, along
+// diagonals: plane pl (0: the block's top plane) of block b at word
+// (b + pl) * npl + pl.  At one step the lanes of a walk segment decode planes
+// pl0 + l of blocks b0 - l, which lie on one diagonal when the blocks'
+// exponents agree: their 8-byte stores form one contiguous row instead of
+// one cache line each.  The on-chip decode reads a block's planes from up to
+// npl rows of the subband's region, which its workgroup reads whole.
+__host__ __device__ constexpr size_t xs_region(int npl) { return size_t(128 + npl) * npl; }
+__host__ __device__ inline int xs_at(int b, int pl, int npl) { return (b + pl) * npl + pl; }
+
 // Past the decode kernel's LDS (faces + DecLds32): the brick's parent
 // subband-0 interior, 16^3 doubles [z][y][x], loaded straight into LDS when
 // the brick starts and copied into V when subband 0 is decoded.
@@ -288,7 +298,7 @@ __device__ __forceinline__ void walk_bin(const DecArgs& a, WalkLds<WS>& W, int b
   const int klo = sb_ok ? W.klo[sbi] : 0, khi = sb_ok ? W.khi[sbi] : -1;
   const int nblk = sb_ok ? a.blk_n[sbi] : 0;
   const int ng = wave_max_i(ngs);
-  const auto xs = gm(a.xs) + ((size_t)bi * 8 + sbi) * 128 * npl;
+  const auto xs = gm(a.xs) + ((size_t)bi * 8 + sbi) * xs_region(npl);
   constexpr bool kStaged = !std::is_same<P, const __attribute__((address_space(1))) uint32_t*>::value;
   for (int gi = 0; gi < ng; ++gi) {
     const int g0 = khi - wdt * gi;
@@ -342,7 +352,7 @@ __device__ __forceinline__ void walk_bin(const DecArgs& a, WalkLds<WS>& W, int b
             int n = n_in;
             const u64 X = decode_plane_win(w32, B, pos, n, 1 << (2 * nd));
             n_out = n;
-            xs[b * npl + (63 - bp)] = X;
+            xs[xs_at(b, 63 - bp, npl)] = X;
             W.nbps[sbi][b] = (unsigned char)(64 - bp);
           }
         } else if (bp > 63) {
@@ -486,13 +496,13 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
 template <int DT>
 __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
                                                const uint32_t* s_blocks, const u64* XS) {
-  // XS: this brick's plane words [subband][128][npl]
+  // XS: this brick's plane words, [subband] regions of xs_region(npl)
   const BrickConst& k = a.k;
   constexpr int npl = DT ? 64 : 32;
   const int lane = p.lane, q = lane >> 4, r = lane & 15;
   const int nblk = a.blk_n[sb];
   const uint32_t* blocks = s_blocks + sb * 128;
-  const auto xs = gm(XS) + (size_t)sb * 128 * npl;
+  const auto xs = gm(XS) + (size_t)sb * xs_region(npl);
   i64* T = &D.T[p.w][0];
   for (int g = p.w; g * 4 < nblk; g += kTh / 64) {
     // this lane's row-mode block
@@ -526,7 +536,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
         const int slot = 2 * pr + (lane >> 5);
         const int blk = g * 4 + slot;
         const int pp = 31 - (lane & 31);
-        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[blk * (DT ? 64 : 32) + pp] : 0ull;
+        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[xs_at(blk, pp, npl)] : 0ull;
         const u64 R = bit_transpose(X, lane);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -544,7 +554,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       for (int qq = 0; qq < 4; ++qq) {
         const int blk = g * 4 + qq;
         const int pp = 63 - lane;
-        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[blk * 64 + pp] : 0ull;
+        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[xs_at(blk, pp, 64)] : 0ull;
         const u64 u = bit_transpose(X, lane);
         const int ndq = nd_s[qq], c = lane;
         if (c < (1 << (2 * ndq))) {

@@ -369,7 +369,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
     const b32::Pos p = b32::pos();
     const int b3[3] = {a.xyz[3 * bi], a.xyz[3 * bi + 1], a.xyz[3 * bi + 2]};
     const int dense = a.dense[bi];
-    const u64* XS = a.xs + (size_t)bi * 8 * 128 * npl;
+    const u64* XS = a.xs + (size_t)bi * 8 * b32::xs_region(npl);
     // the parent interior of subband 0 on its way into LDS while the z-odd
     // subbands decode (16-byte aligned rows only; otherwise read when used)
     double* PB = nullptr;
