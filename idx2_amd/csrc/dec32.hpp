@@ -123,12 +123,15 @@ __device__ __forceinline__ u64 decode_plane_win(P w32, BitBuf& B, u64& pos, int&
     w >>= 1;
     avail -= 1;
     if (!sig) break;  // group insignificant
-    const int z = w ? __ffsll((long long)w) - 1 : 64;
-    int adv;
-    if (z < lim) { n += z; adv = z + 1; }
-    else { n = nv - 1; adv = lim; }
+    // zeros before the next one; bit 63 of w is clear after the shift, so
+    // the sentinel there only stands in for "none" (63 >= lim: the implicit
+    // last one)
+    const int z = __builtin_ctzll(w | (1ull << 63));
+    const bool expl = z < lim;
+    const int adv = expl ? z + 1 : lim;  // <= 63
+    n = expl ? n + z : nv - 1;
     pos += adv;
-    w = adv >= 64 ? 0 : w >> adv;
+    w >>= adv;
     avail -= adv;
     X |= 1ull << n;
     ++n;
