@@ -63,7 +63,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
       const int stop = max(64 - k.endbp, 57 + a.etol - e);
       const int np = stop > 63 ? 0 : 64 - stop;
       bool bypass = false;
-      for (int d = 0; d < 3; ++d) bypass |= (o3[d] % a.spacing[sb][d]) != 0;
+      for (int d = 0; d < 3; ++d) bypass |= (o3[d] & a.spmask[sb][d]) != 0;
       live = np > 0 && !bypass;
     }
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
@@ -131,7 +131,7 @@ __device__ __forceinline__ void inverse_blocks_2d(const DecArgs& a, int sb, int 
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[i0 + 4 * i1 + TDec::c * i2];
       if (nd >= 1) zfp_ilift4(w4[0], w4[1], w4[2], w4[3]);
-      const double scale = 1.0 / pow2d((63 - nd) - 1 - e);
+      const double scale = inv_pow2d((63 - nd) - 1 - e);
       const int vbase = (sg * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1a : 0);
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0)

@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
         int org[3], bd[3], skip, ci;
         unpack_block64(blocks[b], org, bd, &skip, &ci);
         bool bypass = false;
-        for (int d = 0; d < 3; ++d) bypass |= (org[d] % a.spacing[sb][d]) != 0;
+        for (int d = 0; d < 3; ++d) bypass |= (org[d] & a.spmask[sb][d]) != 0;
         const int stop = max(64 - k.endbp, 57 + a.etol - e);
         const int np = stop > 63 ? 0 : 64 - stop;
         if (np == 0 || bypass) continue;
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kThreads) k_decode_level(const DecArgs* __rest
         i64 q = (i64)((us ^ IDX2G_NBMASK) - IDX2G_NBMASK);
         q = zfp_inverse_lanes(q, lane, nd);
         // Dequantize (Function.h:139-149)
-        const double scale = 1.0 / pow2d(prec3[nd] - 1 - e);
+        const double scale = inv_pow2d(prec3[nd] - 1 - e);
         const double f = scale * i64_to_f64(q);
         if (lane < nv) {
           int x = lane % bd[0], y = (lane / bd[0]) % bd[1], z = lane / (bd[0] * bd[1]);

@@ -365,6 +365,11 @@ __device__ __forceinline__ double pow2d(int e) {
   if (e >= -1074) return __longlong_as_double(1ll << (e + 1074));
   return 0.0;
 }
+// 1.0 / pow2d(k) without the division (Dequantize's Scale, Function.h:146):
+// for k in [-1023, 1023] the quotient is exactly 2^-k (2^-1023 subnormal);
+// k > 1023 divides by inf and gives 0; k < -1023 divides by a subnormal or
+// by 0 and gives inf, as pow2d(-k) does.
+__device__ __forceinline__ double inv_pow2d(int k) { return k > 1023 ? 0.0 : pow2d(-k); }
 // (i64)x for |x| < 2^63: truncation toward zero, split exactly into 32-bit
 // halves (t and hi*2^32 are exact, so lo is exact).
 // NaN and |x| >= 2^63 give 0x8000000000000000 (x86's cvttsd2si); computed

@@ -235,7 +235,7 @@ struct DecArgs {
   const uint8_t* exps;
   long long exp_off[8];
   int mask;
-  int spacing[8][3];
+  int spmask[8][3];      // block spacing - 1 per subband and axis (spacings are powers of two)
   int etol;
   // output grid (only when out != nullptr)
   void* out;

@@ -380,6 +380,8 @@ void compute_decode_masks(const Layout& L, const int ds[3], DecodeMasks* m) {
         for (int d = 0; d < 3; ++d) cur0[d] = s3[d];
       for (int d = 0; d < 3; ++d) {
         if (f3[d] % sp[d] == 0) {
+          // sp and s3 are powers of two, so the spacing is one as well (the
+          // decode kernels test block origins against spacing - 1)
           if (s3[d] % sp[d] != 0) m->spacing[l][s][d] = sp[d] / s3[d];
         } else {
           mask = uint8_t(mask & ~(1u << s));

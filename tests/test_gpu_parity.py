@@ -83,9 +83,10 @@ def test_decode_reference_files(ctx, name, spec):
     ds.free()
 
 
-def _roundtrip_case(ctx, oracle_lib, dims, dtype, nlevels, tol, brick, queries, seed=11):
+def _roundtrip_case(ctx, oracle_lib, dims, dtype, nlevels, tol, brick, queries, seed=11, data=None):
     from idx2_amd import api
-    data = synth.make_field(dims, dtype, seed=seed)
+    if data is None:
+        data = synth.make_field(dims, dtype, seed=seed)
     with tempfile.TemporaryDirectory() as wd:
         ds = api.Dataset.encode(ctx, data, name="R", field="f", brick=brick, nlevels=nlevels, tolerance=tol)
         ds.write(wd)
@@ -135,6 +136,43 @@ def test_large_and_anisotropic_bricks_match_oracle(ctx, oracle_lib, dims, dtype,
                     [dict(), dict(downsampling=(1, 1, z)), dict(tolerance=1e-2),
                      dict(first=(5, 3, 0 if z == 0 else 2), last=(dims[0] - 9, dims[1] - 4, max(0, dims[2] - 7)),
                           downsampling=(0, 1, 0))])
+
+
+def test_extreme_magnitudes_match_oracle(ctx, oracle_lib):
+    """float64 bricks scaled near the bottom of the exponent range: at 2^-1000
+    the dequantisation power k = (62 - ndims) - emax exceeds 1023, so
+    Dequantize's scale 1 / 2^k (Function.h:146) divides by inf and is 0, and
+    the quantisation scale 2^k is inf (Function.h:123-127); at 2^-960 the scale
+    is a small normal power.  Tolerance 1e-310 (subnormal) keeps these blocks'
+    planes in the stream.  Files byte-identical with the oracle, decodes
+    bit-identical (signs of zeros included)."""
+    print("case 96^3 f64 extreme magnitudes", flush=True)
+    dims = (96, 96, 96)
+    data = synth.make_field(dims, "float64", seed=5)
+    data[:, :, :32] *= 2.0 ** -1000
+    data[:, :, 32:64] *= 2.0 ** -960
+    data[:, :, 64:] *= 2.0 ** -1000
+    _roundtrip_case(ctx, oracle_lib, dims, "float64", 2, 1e-310, (32, 32, 32),
+                    [dict(), dict(downsampling=(1, 1, 1)), dict(tolerance=1e-300)], data=data)
+
+
+def test_subband_key_range_limit(ctx):
+    """A level-1 brick whose subband spans 2^-1000 .. 2^1000 needs more bit-plane
+    keys than a brick-subband's tables hold (128): the encoder refuses it with
+    NotSupportedInVersion and a message, and the context stays usable."""
+    from idx2_amd import api
+    print("case 96^3 f64 key range beyond the tables", flush=True)
+    dims = (96, 96, 96)
+    data = synth.make_field(dims, "float64", seed=5)
+    data[:, :, :32] *= 2.0 ** -1000
+    data[:, :, 64:] *= 2.0 ** 1000
+    with pytest.raises(api.Idx2Error) as ei:
+        api.Dataset.encode(ctx, data, name="R", field="f", brick=(32, 32, 32), nlevels=2, tolerance=1e-310)
+    assert ei.value.code == 28 and "bit-plane keys" in str(ei.value)
+    ds = api.Dataset.encode(ctx, synth.make_field((64, 64, 64), "float32", seed=1), name="R", field="f",
+                            brick=(32, 32, 32), nlevels=1, tolerance=1e-3)
+    assert ds.decode().shape == (64, 64, 64)
+    ds.free()
 
 
 def test_2d_f64_slice(ctx, oracle_lib):
