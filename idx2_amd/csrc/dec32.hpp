@@ -525,6 +525,26 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       for (int d = 0; d < 3; ++d) bypass |= (o3[d] & a.spmask[sb][d]) != 0;
       live = np > 0 && !bypass;
     }
+    // row mode addressing of the first-axis pass: lane (i1, i2) of its block
+    // writes the row of samples vbase + i0 * s0
+    const int dxy = dx * dy;
+    const int s0 = ux ? 1 : (uy ? dx : dxy);
+    const int s1 = ux ? (uy ? dx : dxy) : (uy ? dxy : 0);
+    const int s2 = (ux && uy && uz) ? dxy : 0;
+    const int i1 = r & 3, i2 = r >> 2;
+    // every block of the table writes its samples (zeros when it is not
+    // decoded), so V needs no zero fill (the blocks tile the subband but for
+    // the subband-0 interior, which holds the parent's data)
+    const bool rowok = bin && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
+    const int n0 = nd >= 1 ? 4 : 1;
+    const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
+    if (__builtin_amdgcn_ballot_w64(live) == 0) {
+      // none of the four blocks is decoded: zeros, no transposes or lifts
+#pragma unroll
+      for (int i0 = 0; i0 < 4; ++i0)
+        if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = 0.0;
+      continue;
+    }
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
     // plane words -> coefficients (bit transpose), inverse order, negabinary;
@@ -594,22 +614,11 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
     wave_sync_lds();
     // row mode: first axis, dequantise, into V (Dequantize, Function.h:139-149)
     {
-      const int dxy = dx * dy;
-      const int s0 = ux ? 1 : (uy ? dx : dxy);
-      const int s1 = ux ? (uy ? dx : dxy) : (uy ? dxy : 0);
-      const int s2 = (ux && uy && uz) ? dxy : 0;
-      const int i1 = r & 3, i2 = r >> 2;
-      // every block of the table writes its samples (zeros when it is not
-      // decoded), so V needs no zero fill (the blocks tile the subband but for
-      // the subband-0 interior, which holds the parent's data)
-      const bool rowok = bin && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
-      const int n0 = nd >= 1 ? 4 : 1;
       i64 w4[4];
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0) w4[i0] = Tq[i0 + 4 * i1 + TDec::c * i2];
       if (nd >= 1) zfp_ilift4(w4[0], w4[1], w4[2], w4[3]);
       const double scale = inv_pow2d((63 - nd) - 1 - e);
-      const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0)
         if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = live ? scale * i64_to_f64(w4[i0]) : 0.0;
