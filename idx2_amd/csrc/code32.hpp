@@ -445,6 +445,35 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
     int emx = rowok ? (DT ? exponent_f64(m) : exponent_f32((float)m)) : -4000;
     emx = row_max(emx);
     const int emax = emx;
+    // block scalars
+    const int stop = max(64 - k.endbp, 57 + a.etol - emax);
+    const int np = skip ? 0 : (stop > 63 ? 0 : 64 - stop);
+    if (r == 0 && bin) {
+      C.E[b] = skip ? kSkip : (short)emax;
+      C.np[b] = (unsigned char)np;
+      if (!skip) {
+        // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
+        const int bdense = kSeg ? seg_dense[o3[2]] : dense;
+        const long long row = a.exp_off[sb] + ((long long)bdense * a.ncoded[sb] + ci) * (DT ? 2 : 1);
+        if (DT) {
+          const uint16_t e16 = (uint16_t)(emax + 1023);
+          gm(a.exps)[row] = (uint8_t)(e16 & 0xff);
+          gm(a.exps)[row + 1] = (uint8_t)(e16 >> 8);
+        } else {
+          gm(a.exps)[row] = (uint8_t)(emax + 127);
+        }
+        if (np > 0) {
+          if constexpr (kSeg) {
+            atomicMax(&C.khi_s[o3[2]], 63 + emax + kBpBiasDev);
+            atomicMin(&C.klo_s[o3[2]], stop + emax + kBpBiasDev);
+          } else {
+            atomicMax(&C.khi, 63 + emax + kBpBiasDev);
+            atomicMin(&C.klo, stop + emax + kBpBiasDev);
+          }
+        }
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(np > 0) == 0) continue;  // no block codes a plane
     const double scale = pow2d((63 - nd) - 1 - emax);
     i64 w4[4];
 #pragma unroll
@@ -478,34 +507,6 @@ __device__ __forceinline__ void block_stage(const EncArgs& a, int sb, int dx, in
       }
     }
     wave_sync_lds();
-    // block scalars
-    const int stop = max(64 - k.endbp, 57 + a.etol - emax);
-    const int np = skip ? 0 : (stop > 63 ? 0 : 64 - stop);
-    if (r == 0 && bin) {
-      C.E[b] = skip ? kSkip : (short)emax;
-      C.np[b] = (unsigned char)np;
-      if (!skip) {
-        // exponent sub-channel (EncodeBrickSubbandExponents, idx2Encode.cpp:41-80)
-        const int bdense = kSeg ? seg_dense[o3[2]] : dense;
-        const long long row = a.exp_off[sb] + ((long long)bdense * a.ncoded[sb] + ci) * (DT ? 2 : 1);
-        if (DT) {
-          const uint16_t e16 = (uint16_t)(emax + 1023);
-          gm(a.exps)[row] = (uint8_t)(e16 & 0xff);
-          gm(a.exps)[row + 1] = (uint8_t)(e16 >> 8);
-        } else {
-          gm(a.exps)[row] = (uint8_t)(emax + 127);
-        }
-        if (np > 0) {
-          if constexpr (kSeg) {
-            atomicMax(&C.khi_s[o3[2]], 63 + emax + kBpBiasDev);
-            atomicMin(&C.klo_s[o3[2]], stop + emax + kBpBiasDev);
-          } else {
-            atomicMax(&C.khi, 63 + emax + kBpBiasDev);
-            atomicMin(&C.klo, stop + emax + kBpBiasDev);
-          }
-        }
-      }
-    }
     // coefficient mode + bit transpose + plane mode
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
                          __builtin_amdgcn_readlane(nd, 32), __builtin_amdgcn_readlane(nd, 48)};
