@@ -341,25 +341,26 @@ __device__ __forceinline__ void walk_bin(const DecArgs& a, WalkLds<WS>& W, int b
         e_next = inn ? W.E[sbi][bn] : kSkip;
         nd_next = W.nd[sbi][inn ? bn : 0];
       }
-      if (kact && inb && e != kSkip) {
-        const int n_in = sl == 0 ? (gi == 0 ? 0 : W.nsave[sbi][b]) : n_left;
-        const int st_in = sl == 0 ? (gi == 0 ? 0 : W.ssave[sbi][b]) : s_left;
-        const int bp = key - kBpBiasDev - e;
-        const int stop = max(64 - endbp, 57 + etol - e);
-        n_out = n_in;
-        stop_out = st_in;
-        if (bp <= 63 && bp >= stop && !st_in) {
-          if (!have) {
-            stop_out = 1;  // missing chunk: the block stops decoding planes
-          } else {
-            int n = n_in;
-            const u64 X = decode_plane_win(w32, B, pos, n, 1 << (2 * nd));
-            n_out = n;
-            xs[xs_at(b, 63 - bp, npl)] = X;
-            W.nbps[sbi][b] = (unsigned char)(64 - bp);
-          }
-        } else if (bp > 63) {
-          n_out = 0;
+      // every lane runs the plane decoder, with no group to decode (nv 0)
+      // where it has no plane at this step: the stream state then stays in
+      // place instead of being copied around a branch (the decoder only
+      // peeks at the lane's current position, inside the padded streams)
+      const bool act = kact && inb && e != kSkip;
+      const int bs = inb ? b : 0;
+      const int n_in = sl == 0 ? (gi == 0 ? 0 : W.nsave[sbi][bs]) : n_left;
+      const int st_in = sl == 0 ? (gi == 0 ? 0 : W.ssave[sbi][bs]) : s_left;
+      const int bp = key - kBpBiasDev - e;
+      const int stop = max(64 - endbp, 57 + etol - e);
+      const bool inplane = act && bp <= 63 && bp >= stop && !st_in;
+      const bool dec = inplane && have;
+      int n = dec ? n_in : 0;
+      const u64 X = decode_plane_win(w32, B, pos, n, dec ? 1 << (2 * nd) : 0);
+      if (act) {
+        n_out = dec ? n : (bp > 63 ? 0 : n_in);
+        stop_out = inplane && !have ? 1 : st_in;  // missing chunk: the block stops decoding planes
+        if (dec) {
+          xs[xs_at(b, 63 - bp, npl)] = X;
+          W.nbps[sbi][b] = (unsigned char)(64 - bp);
         }
         if (sl == wdt - 1 || key == klo) {
           W.nsave[sbi][b] = (unsigned char)n_out;
