@@ -73,20 +73,22 @@ __device__ __forceinline__ void load_parent_lds(const double* pgrid, const int* 
 // reads a different stream, so each load instruction touches 64 cache lines;
 // a 128-bit refill serves two to three planes instead of two 64-bit windows
 // per plane.
+// Positions are bits from the lane's stream pointer (a stream is far below
+// 2^31 bits).
 struct BitBuf {
   u64 b0 = 0, b1 = 0;
-  u64 base = 1ull << 63;  // stream bit of b0's bit 0 (dword aligned); none yet
+  uint32_t base = 1u << 31;  // stream bit of b0's bit 0 (dword aligned); none yet
 };
 
 // 64 bits of the stream at bit pos (words padded by >= 16 bytes)
 template <class P>
-__device__ __forceinline__ u64 bb_peek(P w32, BitBuf& B, u64 pos) {
-  u64 sh = pos - B.base;
+__device__ __forceinline__ u64 bb_peek(P w32, BitBuf& B, uint32_t pos) {
+  uint32_t sh = pos - B.base;
   if (sh > 64) {  // outside the buffer (also pos < base): refill at pos's word
     const auto w = w32 + (pos >> 5);
     B.b0 = (u64)w[0] | ((u64)w[1] << 32);
     B.b1 = (u64)w[2] | ((u64)w[3] << 32);
-    B.base = pos & ~31ull;
+    B.base = pos & ~31u;
     sh = pos & 31;
   }
   return sh == 0 ? B.b0 : sh == 64 ? B.b1 : (B.b0 >> sh) | (B.b1 << (64 - sh));
@@ -96,7 +98,7 @@ __device__ __forceinline__ u64 bb_peek(P w32, BitBuf& B, u64 pos) {
 // A 64-bit window is re-read only when the next group test may need bits
 // beyond it.
 template <class P>
-__device__ __forceinline__ u64 decode_plane_win(P w32, BitBuf& B, u64& pos, int& n, int nv) {
+__device__ __forceinline__ u64 decode_plane_win(P w32, BitBuf& B, uint32_t& pos, int& n, int nv) {
   u64 X = 0;
   u64 w = bb_peek(w32, B, pos);
   int avail = 64;
@@ -310,17 +312,20 @@ __device__ __forceinline__ void walk_bin(const DecArgs& a, WalkLds<WS>& W, int b
     const bool kact = gact && key >= klo;
     const int kk = key - klo;
     bool have = false;
-    u64 pos = 0;
+    uint32_t pos = 0;  // bits from ws
+    P ws = w32;
     if (kStaged) {
       if (kact && kk < kWalkSlots) {
         const uint32_t st = W.sstart[sbi][kk];
         have = st != ~0u;
-        pos = (u64)st * 8;
+        pos = have ? st * 8 : 0;
       }
     } else if (kact && key >= dir_klo && key < dir_klo + dir_nk) {
       const u64 d = dir[((size_t)sbi * dir_nk + (key - dir_klo)) * span + lin];
       have = d != 0;
-      pos = dir_offset(d) * 8;
+      const u64 off = have ? dir_offset(d) : 0;
+      ws = w32 + (off >> 2);
+      pos = uint32_t(off & 3) * 8;
     }
     BitBuf B;
     const int my_steps = gact ? nblk + min(wdt, g0 - klo + 1) : 0;
@@ -354,7 +359,7 @@ __device__ __forceinline__ void walk_bin(const DecArgs& a, WalkLds<WS>& W, int b
       const bool inplane = act && bp <= 63 && bp >= stop && !st_in;
       const bool dec = inplane && have;
       int n = dec ? n_in : 0;
-      const u64 X = decode_plane_win(w32, B, pos, n, dec ? 1 << (2 * nd) : 0);
+      const u64 X = decode_plane_win(ws, B, pos, n, dec ? 1 << (2 * nd) : 0);
       if (act) {
         n_out = dec ? n : (bp > 63 ? 0 : n_in);
         stop_out = inplane && !have ? 1 : st_in;  // missing chunk: the block stops decoding planes
