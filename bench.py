@@ -510,6 +510,42 @@ def plain_bench(ctx, torch, cfg, steps, warmup, seed):
             "dec_achieved": db / (dom_d / 1e3) / 1e9, "raw": raw}
 
 
+def shape_sweep(ctx, torch, seed, steps=3):
+    """Device times of every tunable launch shape (idx2g_tuning), one
+    coordinate at a time from the defaults: the 1024^3 float32 workload and
+    the 2D float64 8192^2 slice (encode and full decode, library device
+    times, mean of `steps` after one warmup) and the 64-field mix (one batch
+    encode + one batch decode, wall time, mean of `steps` after one warmup)."""
+    from idx2_amd import api
+    cands = ([{}] + [{"enc_wg_per_cu": v} for v in (1, 2, 4)] + [{"dec_wg_per_cu": v} for v in (1, 2, 4)]
+             + [{"walk_batch": v} for v in (2048, 8192)] + [{"walk_wide_max": v} for v in (1, 1023, 32767, 1 << 30)])
+    fields, bufs = batched_fields(torch, ctx, seed)
+    outs = [torch.empty_like(b) for b in bufs]
+    rows = []
+    for c in cands:
+        log(f"shape sweep: {c or 'defaults'}")
+        api.set_tuning(0, **c)
+        m3 = plain_bench(ctx, torch, CONFIGS["c2"], steps, 1, seed)
+        m2 = plain_bench(ctx, torch, CONFIGS["c2b"], steps, 1, seed)
+        mix = []
+        for rep in range(steps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dss = api.Dataset.encode_batch(ctx, fields)
+            api.decode_batch(ctx, dss, [o.data_ptr() for o in outs])
+            torch.cuda.synchronize()
+            if rep:
+                mix.append(time.perf_counter() - t0)
+            for ds in dss:
+                ds.free()
+        rows.append({"shape": c or "defaults", "c2_encode_ms": round(m3["enc_ms"], 3),
+                     "c2_decode_ms": round(m3["dec_ms"], 3), "c2b_encode_ms": round(m2["enc_ms"], 3),
+                     "c2b_decode_ms": round(m2["dec_ms"], 3), "mix_ms": round(float(np.mean(mix)) * 1e3, 3)})
+    api.set_tuning(0)
+    del bufs, outs
+    return rows
+
+
 def end_to_end(ctx, torch, cfg, seed, reps=2):
     """Encode of the HBM-resident grid followed by idx2g_dataset_write (host
     zstd of the exponent and address sections, L*.bin and .idx2 files into a
@@ -569,6 +605,8 @@ def main():
     ap.add_argument("--no-traffic", dest="traffic", action="store_false",
                     help="skip the two rocprofv3 --pmc passes that measure the dominant launch's HBM traffic")
     ap.add_argument("--pmc-probe", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--shape-sweep", action="store_true",
+                    help="only time every tunable launch shape on the 1024^3, 8192^2 and 64-field workloads")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one plain device copy of MIB MiB (HBM counter calibration under rocprofv3) and exit")
     args = ap.parse_args()
@@ -591,6 +629,10 @@ def main():
     from idx2_amd import api
 
     ctx = api.Context(0)
+    if args.shape_sweep:
+        print(json.dumps({"shape_sweep": shape_sweep(ctx, torch, args.seed)}))
+        ctx.close()
+        return
     cfg = CONFIGS[args.config]
     tuning = None
     if args.autotune:
