@@ -181,16 +181,26 @@ __device__ __forceinline__ void unstage_subband_even(double (&c)[kN], Lds& S, co
   unstage_faces(S, V, p, hx, hy, 0);
 }
 
-// z-odd subband (hx, hy, 1) from V into the workgroup's parking area
+// One sample (i, j, k) of z-odd subband (hx, hy, 1) to where it waits for
+// the register columns: the workgroup's parking area for i, j < 16
 // (Z1[((hx + 2 hy) * 16 + k) * 256 + j * 16 + i] = column (2i + hx, 2j + hy)
-// at z = 2k + 1); the columns take them after every subband is decoded.
-__device__ __forceinline__ void park_subband_odd(double* Z1, Lds& S, const double* V, const Pos& p, int hx, int hy) {
-  const auto dst = gm(Z1) + (hx + 2 * hy) * 16 * 256;
-  for (int t = p.tid; t < 16 * 256; t += kTh) {
-    const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
-    dst[t] = V[(k * kVY + j) * kVX + i];
+// at z = 2k + 1; Z1s is the subband's region), else the x = 32 / y = 32
+// faces and the corner line.
+__device__ __forceinline__ void park_put(double* Z1s, Lds& S, int hx, int hy, int i, int j, int k, double v) {
+  if (i < 16 && j < 16) gm(Z1s)[k * 256 + j * 16 + i] = v;
+  else if (j < 16) S.ex[(1 + 2 * k) * 32 + hy + 2 * j] = v;
+  else if (i < 16) S.ey[(1 + 2 * k) * 32 + hx + 2 * i] = v;
+  else S.cc[1 + 2 * k] = v;
+}
+
+// A z-odd subband that the query does not decode: zeros where its samples go.
+__device__ __forceinline__ void park_zero(double* Z1s, Lds& S, const Pos& p, int hx, int hy) {
+  for (int t = p.tid; t < 16 * 256; t += kTh) gm(Z1s)[t] = 0.0;
+  for (int t = p.tid; t < 16 * 17; t += kTh) {
+    const int k = t / 17, j = t % 17;
+    if (!hx && (j < 16 || !hy)) park_put(Z1s, S, hx, hy, 16, j, k, 0.0);  // x face, corner
+    if (!hy && j < 16) park_put(Z1s, S, hx, hy, j, 16, k, 0.0);
   }
-  unstage_faces(S, V, p, hx, hy, 1);
 }
 
 __device__ __forceinline__ void unpark_odd(double (&c)[kN], const double* Z1, const Pos& p) {
@@ -502,9 +512,12 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
 // Step 3 for subband sb: plane words -> values in D.v.V (sample (i, j, k) at
 // (k * dy + j) * dx + i), every sample of the table's blocks (the parent's
 // interior is copied for subband 0 below the coarsest level).
-template <int DT>
+// kPark: subband (hx, hy, 1), every sample straight to park_put (no V, no
+// workgroup barrier: a wave's blocks touch nothing another wave reads).
+template <int DT, bool kPark = false>
 __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx, int dy, DecLds32& D, const Pos& p,
-                                               const uint32_t* s_blocks, const u64* XS) {
+                                               const uint32_t* s_blocks, const u64* XS, double* Z1s = nullptr,
+                                               Lds* S = nullptr, int hx = 0, int hy = 0) {
   // XS: this brick's plane words, [subband] regions of xs_region(npl)
   const BrickConst& k = a.k;
   constexpr int npl = DT ? 64 : 32;
@@ -544,11 +557,23 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
     const bool rowok = bin && (nd == 3 || (nd == 2 && i2 == 0) || (nd <= 1 && r == 0));
     const int n0 = nd >= 1 ? 4 : 1;
     const int vbase = (o3[2] * dy + o3[1]) * dx + o3[0] + (nd >= 2 ? i1 * s1 : 0) + (nd == 3 ? i2 * s2 : 0);
+    // kPark: the row's first sample (pi, pj, pk) and its axis ra (the same
+    // samples as vbase + i0 * s0)
+    const int ra = ux ? 0 : (uy ? 1 : 2);
+    const bool a1y = ux && uy;  // second unit axis y (else z)
+    const int pi = o3[0], pj = o3[1] + (nd >= 2 && a1y ? i1 : 0),
+              pk = o3[2] + (nd >= 2 && !a1y ? i1 : 0) + (nd == 3 ? i2 : 0);
+    auto put = [&](int i0, double v) {
+      if constexpr (kPark)
+        park_put(Z1s, *S, hx, hy, pi + (ra == 0 ? i0 : 0), pj + (ra == 1 ? i0 : 0), pk + (ra == 2 ? i0 : 0), v);
+      else
+        D.v.V[vbase + i0 * s0] = v;
+    };
     if (__builtin_amdgcn_ballot_w64(live) == 0) {
       // none of the four blocks is decoded: zeros, no transposes or lifts
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0)
-        if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = 0.0;
+        if (rowok && i0 < n0) put(i0, 0.0);
       continue;
     }
     const int nd_s[4] = {__builtin_amdgcn_readlane(nd, 0), __builtin_amdgcn_readlane(nd, 16),
@@ -627,7 +652,7 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       const double scale = inv_pow2d((63 - nd) - 1 - e);
 #pragma unroll
       for (int i0 = 0; i0 < 4; ++i0)
-        if (rowok && i0 < n0) D.v.V[vbase + i0 * s0] = live ? scale * i64_to_f64(w4[i0]) : 0.0;
+        if (rowok && i0 < n0) put(i0, live ? scale * i64_to_f64(w4[i0]) : 0.0);
     }
     wave_sync_lds();
   }

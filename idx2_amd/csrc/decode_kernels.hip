@@ -387,14 +387,18 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
     }
     __syncthreads();
     IDX2G_PHASE(16);
+    // z-odd subbands straight into the parking area and the faces: no V
+    // staging and no barrier between them
     for (int sb = 0; sb < 8; ++sb) {
       if (k.sb[sb].f[2] != 1) continue;
-      decode_subband_b32<DT>(a, sb, b3, D, p, s_blocks, XS);
-      IDX2G_PHASE(17);
-      b32::park_subband_odd(Z1, S, D.v.V, p, k.sb[sb].f[0], k.sb[sb].f[1]);
-      __syncthreads();
-      IDX2G_PHASE(18);
+      const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
+      double* Z1s = Z1 + (hx + 2 * hy) * 16 * 256;
+      if (!((a.mask >> sb) & 1)) b32::park_zero(Z1s, S, p, hx, hy);
+      else b32::inverse_blocks<DT, true>(a, sb, kVX, kVY, D, p, s_blocks, XS, Z1s, &S, hx, hy);
     }
+    IDX2G_PHASE(17);
+    __syncthreads();
+    IDX2G_PHASE(18);
     double c[b32::kN];
     for (int sb = 0; sb < 8; ++sb) {
       if (k.sb[sb].f[2] != 0) continue;

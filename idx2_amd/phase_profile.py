@@ -28,7 +28,8 @@ PHASES = [
 ]
 DEC_PHASES = [
     (16, "decode: setup (exponents, plane counts, parent rows issued)"),
-    (17, "decode: z-odd subbands, inverse block stage"), (18, "decode: z-odd subbands parked"),
+    (17, "decode: z-odd subbands, inverse block stage into the parking area"),
+    (18, "decode: z-odd subbands, wait for every wave"),
     (19, "decode: z-even subbands, inverse block stage"), (20, "decode: z-even subbands into the columns"),
     (21, "decode: unpark + inverse weights"), (22, "decode: inverse CDF5/3 (Z, Y, X)"),
     (23, "decode: crop into the output"),
