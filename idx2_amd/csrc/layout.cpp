@@ -314,20 +314,46 @@ void Layout::level_bricks(int l, std::vector<uint64_t>* lin, std::vector<int>* x
   // walk the linear ids in order and keep those inside the level's brick
   // grid: the result comes out sorted, no per-brick search or sort
   const int n = nbricks[l][0] * nbricks[l][1] * nbricks[l][2];
-  lin->clear();
-  xyz->clear();
-  lin->reserve(n);
-  xyz->reserve(3 * (size_t)n);
+  lin->resize(n);
+  xyz->resize(3 * (size_t)n);
   const uint64_t span = linear_span(l);
   const int nb = (int)bstr[l].size();
-  int axis[64];
+  // bit i of the brick string (most significant first) is bit `rank[i]` of
+  // coordinate axis[i], rank[i] = the later bits of the same axis; a table
+  // per byte of the id holds that byte's share of (x, y, z)
+  int axis[64], rank[64];
   for (int i = 0; i < nb; ++i) axis[i] = bstr[l][i] - 'X';
-  for (uint64_t id = 0; id < span && (int)lin->size() < n; ++id) {
-    int b3[3] = {0, 0, 0};
-    for (int i = 0; i < nb; ++i) b3[axis[i]] = (b3[axis[i]] << 1) | int((id >> (nb - i - 1)) & 1);
-    if (b3[0] >= nbricks[l][0] || b3[1] >= nbricks[l][1] || b3[2] >= nbricks[l][2]) continue;
-    lin->push_back(id);
-    xyz->insert(xyz->end(), b3, b3 + 3);
+  for (int i = 0; i < nb; ++i) {
+    rank[i] = 0;
+    for (int j = i + 1; j < nb; ++j) rank[i] += axis[j] == axis[i];
+  }
+  const int nbytes = (nb + 7) / 8;
+  std::vector<int> tab((size_t)nbytes * 256 * 3, 0);
+  for (int by = 0; by < nbytes; ++by)
+    for (int v = 0; v < 256; ++v)
+      for (int bit = 0; bit < 8; ++bit) {
+        const int pos = by * 8 + bit, i = nb - 1 - pos;  // id bit pos = string bit i
+        if (pos < nb && ((v >> bit) & 1)) tab[((size_t)by * 256 + v) * 3 + axis[i]] |= 1 << rank[i];
+      }
+  const int* T = tab.data();
+  const int n0 = nbricks[l][0], n1 = nbricks[l][1], n2 = nbricks[l][2];
+  uint64_t* L = lin->data();
+  int* X = xyz->data();
+  int k = 0;
+  for (uint64_t id = 0; id < span && k < n; ++id) {
+    int x = 0, y = 0, z = 0;
+    for (int by = 0; by < nbytes; ++by) {
+      const int* t = T + ((size_t)by * 256 + ((id >> (8 * by)) & 255)) * 3;
+      x |= t[0];
+      y |= t[1];
+      z |= t[2];
+    }
+    if (x >= n0 || y >= n1 || z >= n2) continue;
+    L[k] = id;
+    X[3 * k] = x;
+    X[3 * k + 1] = y;
+    X[3 * k + 2] = z;
+    ++k;
   }
 }
 
