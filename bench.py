@@ -237,7 +237,9 @@ def _serve_run(torch, sets, reqs, rate, slots, budget, max_out):
     t_next = t0
     stalls = 0
     try:
-        for k, q, gap in reqs:
+        for i, (k, q, gap) in enumerate(reqs):
+            if i and i % 2000 == 0:
+                log(f"serving: {i} of {len(reqs)} requests submitted")
             ds, _, dt = sets[k]
             t_next += gap * 400.0 / rate  # the stream's gaps are drawn at 400/s
             now = time.perf_counter()
@@ -251,8 +253,27 @@ def _serve_run(torch, sets, reqs, rate, slots, budget, max_out):
                     break
                 stalls += 1
                 time.sleep(0.001)
-            buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
-                              dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+            released = False
+            while True:
+                try:
+                    buf = torch.empty(max(1, nbytes // (8 if dt == "float64" else 4)),
+                                      dtype=torch.float64 if dt == "float64" else torch.float32, device="cuda")
+                    break
+                except torch.OutOfMemoryError:
+                    # device memory is short (admitted requests, cached levels, the
+                    # library's kept blocks): wait for a pending result to free its
+                    # buffer, or once nothing is pending hand the kept blocks back
+                    stalls += 1
+                    with lock:
+                        busy = bool(pending)
+                    if busy:
+                        time.sleep(0.001)
+                    elif released:
+                        raise
+                    else:
+                        api.release_device_cache(0)
+                        torch.cuda.empty_cache()
+                        released = True
             rid = server.submit(ds, buf.data_ptr(), out_owner=buf, **q)
             with lock:
                 pending.append((rid, buf, nbytes))
@@ -277,7 +298,7 @@ def _serve_run(torch, sets, reqs, rate, slots, budget, max_out):
             "cache_bytes": cnt["cache_bytes"], "client_stalls": stalls}
 
 
-def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=48 << 30,
+def serve_bench(ctx, torch, n_requests, rate, seed, slots=4, budget=200 << 30, max_out=32 << 30,
                 sweep=(800.0, 1600.0, 3200.0, 6400.0), sweep_requests=2000):
     """The serving loop on its three resident datasets: n_requests of the
     seeded request stream at `rate`, then a sweep of higher rates
@@ -729,6 +750,10 @@ def main():
         res["end_to_end"] = end_to_end(ctx, torch, cfg, args.seed)
     if args.serve_requests > 0:
         log(f"serving loop: {args.serve_requests} requests at {args.serve_rate}/s")
+        # the device blocks earlier legs left in the library's and torch's caches
+        # go back first: the serving loop's budget and client buffers need the room
+        api.release_device_cache(0)
+        torch.cuda.empty_cache()
         res["serving"] = serve_bench(ctx, torch, args.serve_requests, args.serve_rate, args.seed)
     if args.progressive:
         log("progressive sweep")

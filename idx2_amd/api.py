@@ -78,7 +78,7 @@ EXPORTS = ["idx2g_create", "idx2g_destroy", "idx2g_last_error_message", "idx2g_e
            "idx2g_get_tuning", "idx2g_decode_noalloc", "idx2g_server_counters",
            "idx2g_decode_bricks", "idx2g_brick_pool_count", "idx2g_brick_pool_key", "idx2g_brick_pool_find",
            "idx2g_brick_pool_lookup", "idx2g_brick_pool_resolution", "idx2g_brick_pool_copy", "idx2g_brick_pool_write",
-           "idx2g_brick_pool_free"]
+           "idx2g_brick_pool_free", "idx2g_release_device_cache"]
 
 _lib = None
 
@@ -142,6 +142,8 @@ def lib():
                                      ctypes.POINTER(ctypes.c_double)]
         L.idx2g_set_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(Tuning)]
         L.idx2g_get_tuning.argtypes = [ctypes.c_int, ctypes.POINTER(Tuning)]
+        L.idx2g_release_device_cache.argtypes = [ctypes.c_int]
+        L.idx2g_release_device_cache.restype = ctypes.c_int64
         L.idx2g_encode_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(EncodeParams), ctypes.POINTER(vp),
                                          ctypes.c_int, ctypes.POINTER(vp)]
         L.idx2g_decode_batch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp), ctypes.POINTER(Query),
@@ -404,6 +406,15 @@ def autotune(ctx):
 def set_tuning(device=0, enc_wg_per_cu=0, dec_wg_per_cu=0, walk_batch=0, walk_wide_max=0):
     t = Tuning(enc_wg_per_cu, dec_wg_per_cu, walk_batch, walk_wide_max)
     _check(lib().idx2g_set_tuning(device, ctypes.byref(t)))
+
+
+def release_device_cache(device=0):
+    """Hand the device blocks the library keeps for reuse back to the runtime
+    (idx2g_release_device_cache); returns the bytes released."""
+    r = lib().idx2g_release_device_cache(device)
+    if r < 0:
+        raise Idx2Error(-r, lib().idx2g_last_error_message().decode(errors="replace"))
+    return r
 
 
 def get_tuning(device=0):

@@ -211,6 +211,14 @@ void dev_free(void* p) {
   c.live.erase(it);
 }
 
+size_t dev_cache_release(int device) {
+  DevCache& c = dev_cache();
+  std::lock_guard<std::mutex> g(c.mu);
+  const size_t held = c.cached[device];
+  trim(c, device, 0);
+  return held - c.cached[device];
+}
+
 // Pinned host blocks kept for reuse (file writing stages whole chunk blobs
 // through them; pinning hundreds of MB per call would cost more than the
 // copy).  Blocks are returned by pinned_free and reused by the next request

@@ -117,6 +117,9 @@ void set_thread_arena(ScratchArena* a);
 // library call that freed them has synchronised its stream.
 hipError_t dev_alloc(void** p, size_t bytes);
 void dev_free(void* p);
+// Hands every kept free block of `device` back to the runtime; returns the
+// bytes released.
+size_t dev_cache_release(int device);
 // Pinned host memory through a process-wide cache of freed blocks.
 hipError_t pinned_alloc(void** p, size_t bytes);
 void pinned_free(void* p);

@@ -240,6 +240,17 @@ int idx2g_autotune(idx2g_ctx* ctx, idx2g_tuning* out, double* best_ms, double* d
 int idx2g_set_tuning(int hip_device, const idx2g_tuning* t);
 int idx2g_get_tuning(int hip_device, idx2g_tuning* t);
 
+/* ---- device memory held for reuse ----
+ * No counterpart in the reference (it allocates host memory per call).  The
+ * library keeps the device blocks its calls free for its next calls
+ * (returning them to the runtime costs a device synchronisation each, so it
+ * only does so itself when an allocation fails).  A process that shares the
+ * GPU with another allocator (a framework's caching allocator, say) calls
+ * this when its library calls are done to hand every kept block of the
+ * device back; later calls allocate again.  Returns the bytes handed back,
+ * or a negative error code. */
+int64_t idx2g_release_device_cache(int hip_device);
+
 /* ---- decode serving loop ----
  * No counterpart in the reference (its decoder serves one query per call,
  * Source/Core/idx2Decode.cpp:326-492, Source/idx2.h:43-44): a queue of decode

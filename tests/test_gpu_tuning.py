@@ -54,3 +54,20 @@ def test_autotune_picks_valid_shapes(ctx):
     assert 0 < best <= dflt
     assert api.get_tuning(0) == shapes
     api.set_tuning(0)
+
+
+def test_release_device_cache_keeps_results(ctx):
+    """idx2g_release_device_cache hands the kept free blocks back (a
+    non-negative byte count; an invalid device is an error), and the calls
+    after it allocate again and give the same bytes and grids."""
+    from idx2_amd import api
+    print("case release_device_cache", flush=True)
+    dims, dt, nl, brick = CASES[0]
+    a = _run(ctx, dims, dt, nl, brick)
+    assert api.release_device_cache(0) > 0  # the encode/decode freed their work buffers
+    assert api.release_device_cache(0) == 0
+    with pytest.raises(api.Idx2Error):
+        api.release_device_cache(1 << 20)
+    b = _run(ctx, dims, dt, nl, brick)
+    for w, g in zip(a, b):
+        assert np.array_equal(w.view(np.uint8), g.view(np.uint8))

@@ -10,7 +10,7 @@ from conftest import ROOT
 def test_library_exports_header_symbols():
     from idx2_amd import api
     header = open(os.path.join(ROOT, "include", "idx2g.h")).read()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(idx2g_\w+)\s*\(", header, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(idx2g_\w+)\s*\(", header, re.M))
     assert declared, "no declarations parsed"
     lib = ctypes.CDLL(api.LIB_PATH)
     for sym in sorted(declared):
