@@ -585,13 +585,33 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
     const bool narrow = DT == 0 || max(max(__builtin_amdgcn_readlane(nbq, 0), __builtin_amdgcn_readlane(nbq, 16)),
                                        max(__builtin_amdgcn_readlane(nbq, 32), __builtin_amdgcn_readlane(nbq, 48))) <= 32;
     if (narrow) {
+      // both plane words first (no branch around the loads: an absent plane
+      // reads word 0 of the region and is zeroed), so that their latencies
+      // overlap
+      u64 Xp[2];
+      bool okp[2];
+      const u64* ap[2];
 #pragma unroll
       for (int pr = 0; pr < 2; ++pr) {
-        const int slot = 2 * pr + (lane >> 5);
-        const int blk = g * 4 + slot;
+        const int blk = g * 4 + 2 * pr + (lane >> 5);
         const int pp = 31 - (lane & 31);
-        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[xs_at(blk, pp, npl)] : 0ull;
-        const u64 R = bit_transpose(X, lane);
+        okp[pr] = blk < nblk && pp < D.nbps[sb][blk < nblk ? blk : 0];
+        ap[pr] = XS + (size_t)sb * xs_region(npl) + (okp[pr] ? xs_at(blk, pp, npl) : 0);
+      }
+      // the compiler, at its register limit here, would wait for the first
+      // load before issuing the second; both in one block, waited for in it
+      asm volatile(
+          "global_load_dwordx2 %0, %2, off\n\t"
+          "global_load_dwordx2 %1, %3, off\n\t"
+          "s_waitcnt vmcnt(0)"
+          : "=&v"(Xp[0]), "=&v"(Xp[1])
+          : "v"(ap[0]), "v"(ap[1])
+          : "memory");
+#pragma unroll
+      for (int pr = 0; pr < 2; ++pr) Xp[pr] = okp[pr] ? Xp[pr] : 0ull;
+#pragma unroll
+      for (int pr = 0; pr < 2; ++pr) {
+        const u64 R = bit_transpose(Xp[pr], lane);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const int sl = 2 * pr + h, ndq = nd_s[sl];
@@ -604,12 +624,18 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
         }
       }
     } else {
+      u64 Xq[4];
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) {
         const int blk = g * 4 + qq;
         const int pp = 63 - lane;
-        const u64 X = (blk < nblk && pp < D.nbps[sb][blk]) ? xs[xs_at(blk, pp, 64)] : 0ull;
-        const u64 u = bit_transpose(X, lane);
+        const bool ok = blk < nblk && pp < D.nbps[sb][blk < nblk ? blk : 0];
+        const u64 v = xs[ok ? xs_at(blk, pp, 64) : 0];
+        Xq[qq] = ok ? v : 0ull;
+      }
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) {
+        const u64 u = bit_transpose(Xq[qq], lane);
         const int ndq = nd_s[qq], c = lane;
         if (c < (1 << (2 * ndq))) {
           const int dst = ndq == 3 ? D.p3[c] : ndq == 2 ? D.p2[c] : c;
