@@ -704,10 +704,13 @@ __device__ __forceinline__ void code_subband(const EncArgs& a, int sb, int seq, 
 
 template <int DT>
 __device__ __forceinline__ void code_brick(const EncArgs& a, const double (&c)[kN], Lds& S, CodeLds& C, const Pos& p,
-                                           int dense, u64* XG, const uint32_t* s_blocks, ArenaCursor& cur) {
+                                           int dense, u64* XG, const uint32_t* s_blocks, ArenaCursor& cur,
+                                           const unsigned char* s_perm) {
+  // s_perm: the job's coefficient orders, kept in the kernel's own LDS (CodeLds
+  // shares LDS with the transform, so it takes them per brick)
   const BrickConst& k = a.k;
-  if (p.tid < 64) C.p3[p.tid] = (unsigned char)TEnc::pos(a.perm.p3[p.tid]);  // positions in the TEnc layout
-  if (p.tid < 16) C.p2[p.tid] = a.perm.p2[p.tid];
+  if (p.tid < 64) C.p3[p.tid] = s_perm[p.tid];  // positions in the TEnc layout
+  if (p.tid < 16) C.p2[p.tid] = s_perm[64 + p.tid];
   if (p.tid < 256) {
     uint32_t e = 0;
     int o = 0;

@@ -625,14 +625,26 @@ __device__ __forceinline__ void inverse_blocks(const DecArgs& a, int sb, int dx,
       }
     } else {
       u64 Xq[4];
+      bool okq[4];
+      const u64* aq[4];
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) {
         const int blk = g * 4 + qq;
         const int pp = 63 - lane;
-        const bool ok = blk < nblk && pp < D.nbps[sb][blk < nblk ? blk : 0];
-        const u64 v = xs[ok ? xs_at(blk, pp, 64) : 0];
-        Xq[qq] = ok ? v : 0ull;
+        okq[qq] = blk < nblk && pp < D.nbps[sb][blk < nblk ? blk : 0];
+        aq[qq] = XS + (size_t)sb * xs_region(npl) + (okq[qq] ? xs_at(blk, pp, 64) : 0);
       }
+      asm volatile(
+          "global_load_dwordx2 %0, %4, off\n\t"
+          "global_load_dwordx2 %1, %5, off\n\t"
+          "global_load_dwordx2 %2, %6, off\n\t"
+          "global_load_dwordx2 %3, %7, off\n\t"
+          "s_waitcnt vmcnt(0)"
+          : "=&v"(Xq[0]), "=&v"(Xq[1]), "=&v"(Xq[2]), "=&v"(Xq[3])
+          : "v"(aq[0]), "v"(aq[1]), "v"(aq[2]), "v"(aq[3])
+          : "memory");
+#pragma unroll
+      for (int qq = 0; qq < 4; ++qq) Xq[qq] = okq[qq] ? Xq[qq] : 0ull;
 #pragma unroll
       for (int qq = 0; qq < 4; ++qq) {
         const u64 u = bit_transpose(Xq[qq], lane);

@@ -340,6 +340,7 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
   extern __shared__ __align__(16) unsigned char lds_raw[];
   __shared__ uint32_t s_blocks[8 * 128];
   __shared__ double w8[8];
+  __shared__ unsigned char s_perm[80];  // coefficient order: TDec positions (64), then 2D order (16)
   b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
   b32::DecLds32& D = *reinterpret_cast<b32::DecLds32*>(&S.u);
   constexpr int npl = DT ? 64 : 32;
@@ -361,6 +362,8 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
         const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
         w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.winv[s];
       }
+      if (threadIdx.x < 64) s_perm[threadIdx.x] = (unsigned char)TDec::pos(a.perm.p3[threadIdx.x]);
+      if (threadIdx.x < 16) s_perm[64 + threadIdx.x] = a.perm.p2[threadIdx.x];
       cj = j;
       __syncthreads();
     }
@@ -377,8 +380,10 @@ __global__ void __launch_bounds__(b32::kTh) k_decode_b32(const DecArgs* __restri
       PB = reinterpret_cast<double*>(lds_raw + b32::dec_parent_off());
       b32::load_parent_lds(a.pgrid, a.pgdims, b3, PB, p);
     }
-    if (p.tid < 64) D.p3[p.tid] = (unsigned char)TDec::pos(a.perm.p3[p.tid]);  // positions in the TDec layout
-    if (p.tid < 16) D.p2[p.tid] = a.perm.p2[p.tid];
+    // the job's coefficient orders (positions in the TDec layout), from the
+    // kernel's own LDS: DecLds32 shares LDS with the transform
+    if (p.tid < 64) D.p3[p.tid] = s_perm[p.tid];
+    if (p.tid < 16) D.p2[p.tid] = s_perm[64 + p.tid];
     // exponents and decoded plane counts of the brick's blocks
     {
       const int sb = p.tid >> 7, b = p.tid & 127;

@@ -368,6 +368,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
   __shared__ EncShared sh;
   __shared__ double w8[8];
   __shared__ uint32_t s_blocks[8 * 128];
+  __shared__ unsigned char s_perm[80];  // coefficient order: TEnc positions (64), then 2D order (16)
   __shared__ b32::ArenaCursor cur;
   b32::Lds& S = *reinterpret_cast<b32::Lds*>(lds_raw);
   if (threadIdx.x == 0) cur = b32::ArenaCursor{0, 0, 0, 0, 0};
@@ -389,6 +390,8 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
         const int idx = k.sb[s].f[0] + 2 * k.sb[s].f[1] + 4 * k.sb[s].f[2];
         w8[idx] = (s == 0 && !a.coarsest) ? 1.0 : a.wfwd[s];
       }
+      if (threadIdx.x < 64) s_perm[threadIdx.x] = (unsigned char)TEnc::pos(a.perm.p3[threadIdx.x]);
+      if (threadIdx.x < 16) s_perm[64 + threadIdx.x] = a.perm.p2[threadIdx.x];
       cj = j;
       __syncthreads();
     }
@@ -438,7 +441,7 @@ __global__ void __launch_bounds__(b32::kTh) k_encode_b32(const EncArgs* __restri
     b32::weigh(c, S, p, w8);
     IDX2G_PHASE(11);
     b32::code_brick<DT>(a, c, S, *reinterpret_cast<b32::CodeLds*>(&S.u), p, dense,
-                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride), s_blocks, cur);
+                    reinterpret_cast<u64*>(a.scratch + (size_t)blockIdx.x * a.wstride), s_blocks, cur, s_perm);
     __syncthreads();
     IDX2G_PHASE(15);
   }
