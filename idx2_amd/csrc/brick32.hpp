@@ -254,7 +254,9 @@ __device__ __forceinline__ void xpass(double (&c)[kN], Lds& S, const Pos& p) {
 template <int MODE>
 __device__ __forceinline__ void ypass(double (&c)[kN], Lds& S, const Pos& p) {
   constexpr int nz = MODE == kExt ? 32 : 33;
-  const int x = p.x, w = p.w;
+  // w (the wave, i.e. the pair of rows) is uniform across the wave: in a
+  // scalar register the row-15 face work is a branch taken by one wave only
+  const int x = p.x, w = __builtin_amdgcn_readfirstlane(p.w);
   // x = 32 column lines
   for (int z = w; z < nz; z += kTh / 64) {
     double* q = p.lane < 32 ? &S.ex[z * 32 + p.lane] : &S.cc[z];
