@@ -221,7 +221,7 @@ __device__ __forceinline__ Pos pos() {
   p.tid = threadIdx.x;
   asm volatile("" : "+v"(p.tid));
   p.lane = p.tid & 63;
-  p.w = p.tid >> 6;
+  p.w = __builtin_amdgcn_readfirstlane(p.tid >> 6);  // uniform: the wave
   p.x = p.tid & 31;
   p.y = p.tid >> 5;
   p.orow = p.lane >= 32;
