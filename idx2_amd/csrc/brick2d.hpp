@@ -37,7 +37,7 @@ __device__ __forceinline__ Pos2 pos2() {
   p.tid = threadIdx.x;
   asm volatile("" : "+v"(p.tid));
   p.lane = p.tid & 63;
-  p.w = p.tid >> 6;
+  p.w = __builtin_amdgcn_readfirstlane(p.tid >> 6);  // uniform: the wave
   p.x = p.tid & 31;
   p.slot = p.tid >> 5;
   return p;

@@ -393,7 +393,7 @@ __device__ __forceinline__ void walk_wg(const DecArgs& a, WalkLds<WS>& W, int bi
   constexpr int NSB = WS::nsb, MB = WS::mb, TH = WS::th;
   const BrickConst& k = a.k;
   const int npl = k.dtype ? 64 : 32;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);  // w: uniform
   const u64 lin = a.lin[bi];
   const int dense = a.dense[bi];
   if (tid < NSB) { W.klo[tid] = 0x7fffffff; W.khi[tid] = -0x7fffffff; }
