@@ -346,9 +346,16 @@ __device__ __forceinline__ void park_odd(const double (&c)[kN], double* Z1, cons
 __device__ __forceinline__ void stage_subband_odd(const double* Z1, const Lds& S, double* V, const Pos& p, int hx,
                                                   int hy) {
   const auto src = gm(Z1) + (hx + 2 * hy) * 16 * 256;
-  for (int t = p.tid; t < 16 * 256; t += kTh) {
+  static_assert(16 * 256 == 4 * kTh, "four samples per thread");
+  // the four loads first, so that their latencies overlap
+  double v[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) v[r] = src[p.tid + r * kTh];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int t = p.tid + r * kTh;
     const int k = t >> 8, j = (t >> 4) & 15, i = t & 15;
-    V[(k * kVY + j) * kVX + i] = src[t];
+    V[(k * kVY + j) * kVX + i] = v[r];
   }
   stage_faces(S, V, p, hx, hy, 1);
 }
