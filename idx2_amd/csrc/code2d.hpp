@@ -60,7 +60,16 @@ __device__ __forceinline__ void code_group(const EncArgs& a, int sb, int G, cons
   const int hx = k.sb[sb].f[0], hy = k.sb[sb].f[1];
   const int dx = 17 - hx, dy = 17 - hy;
   const int nblk = G * nb1;
-  for (int t = p.tid; t < G * dx * dy; t += kTh) C.v.V[t] = src[t];
+  {  // two loads in flight per thread (the loop would wait for each in turn)
+    const int n = G * dx * dy;
+    for (int t0 = p.tid; t0 < n; t0 += 2 * kTh) {
+      const int t1 = t0 + kTh;
+      const double v0 = src[t0];
+      const double v1 = t1 < n ? src[t1] : 0.0;
+      C.v.V[t0] = v0;
+      if (t1 < n) C.v.V[t1] = v1;
+    }
+  }
   for (int b = p.tid; b < nblk; b += kTh) {
     const int g = b / nb1;
     C.gblk[b] = sb_blocks[b - g * nb1] | ((uint32_t)g << 10);
